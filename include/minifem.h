@@ -1,0 +1,197 @@
+/*
+ * minifem — MI355X-native Mini-FEM assembly + block-Jacobi preconditioner.
+ *
+ * C ABI of libminifem.so. Plain pointers and sizes only; every function returns an
+ * int status (MF_OK = 0, negative on error) unless it returns a value. Index
+ * conventions are the reference's: elemToNode and CSR column ids are 1-based, CSR row
+ * offsets and CSR positions are 0-based, ela blocks are 9 doubles row-major, and
+ * checkBounds is markf(N,3) in Fortran order (checkBounds[k*N + i]).
+ *
+ * Host setup functions run on the CPU (they are the reference's one-time setup).
+ * Per-iteration functions run on the GPU: pointer arguments documented "device" must
+ * be device memory (hipMalloc or a torch cuda tensor); `stream` is a hipStream_t
+ * passed as void* (NULL = the default stream). Device entry points return after
+ * enqueueing work unless stated otherwise.
+ */
+#ifndef MINIFEM_H
+#define MINIFEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MF_OK 0
+#define MF_ERR_ARG (-1)      /* bad argument (null pointer, negative size, bad op id) */
+#define MF_ERR_IO (-2)       /* file missing, truncated or unwritable */
+#define MF_ERR_COLORS (-3)   /* greedy coloring needs more than MF_MAX_COLOR colors */
+#define MF_ERR_NNZ (-4)      /* CSR pattern larger than the nnz capacity given */
+#define MF_ERR_HIP (-5)      /* HIP runtime error or no GPU */
+#define MF_ERR_NOMEM (-6)    /* host or device allocation failed */
+#define MF_ERR_BUDGET (-7)   /* request rejected by the HBM budget */
+#define MF_ERR_STATE (-8)    /* object used in the wrong state */
+
+#define MF_MAX_COLOR 128     /* src/coloring.cc:22 */
+#define MF_OP_LAP 0          /* operatorID, src/main.cc:131-138 */
+#define MF_OP_ELA 1
+#define MF_PREC_FP64 0       /* plain FP64 (the drop-in) */
+#define MF_PREC_MIXED 1      /* FP32 element products, FP64 Jacobian/scatter/inverse */
+
+const char *mf_status_string(int status);
+int mf_version(void);
+
+/* ---------------------------------------------------------------------------------
+ * Mesh and reference-norm files (the reference's native-endian binary format).
+ * ------------------------------------------------------------------------------- */
+
+/* Header of src/IO.cc:75-80: {nbElem, nbNodes, nbEdges, nbIntf, nbIntfNodes,
+ * nbBoundNodes}. Replaces the first half of read_input_data (src/IO.cc:61-80). */
+int mf_read_header(const char *path, int header[6]);
+
+/* Body of read_input_data (src/IO.cc:82-95) into caller-allocated arrays sized from
+ * the header: coord[3N], elemToNode[4nE], neighborsList[3*max(nbIntf,1)],
+ * intfIndex[nbIntf+1], intfNodes[nbIntfNodes], boundNodesCode[N]. */
+int mf_read_input_data(const char *path, double *coord, int *elemToNode,
+                       int *neighborsList, int *intfIndex, int *intfNodes,
+                       int *boundNodesCode);
+
+/* store_input_data_ (src/IO.cc:99-127), one rank: writes the header given plus the
+ * arrays, to `path` (the reference writes "inputs_<np>_<rank>" in the cwd). */
+int mf_store_input_data(const char *path, const int header[6], const double *coord,
+                        const int *elemToNode, const int *neighborsList,
+                        const int *intfIndex, const int *intfNodes,
+                        const int *boundNodesCode);
+
+/* read_ref_assembly (src/IO.cc:26-39): two numbers, values norm then prec norm. */
+int mf_read_ref_assembly(const char *path, double *matrixNorm, double *precNorm);
+
+/* store_ref_assembly_ (src/IO.cc:42-58): writes both norms with 17 digits. */
+int mf_store_ref_assembly(const char *path, double matrixNorm, double precNorm);
+
+/* ---------------------------------------------------------------------------------
+ * Synthetic jittered-cube meshes (no reference counterpart: the reference reads
+ * external meshes). n nodes per side, Kuhn 6-tet split of each hex, hexes and nodes
+ * i-fastest, interior nodes jittered uniformly by +-jitter*h, optional seeded node
+ * shuffle. Boundary codes: x=0 -> 52, y=0 -> 53, z=0 -> 54, x/y/z=1 -> 10, with
+ * precedence 10 > 52 > 53 > 54.
+ * ------------------------------------------------------------------------------- */
+
+/* nbElem = 6(n-1)^3, nbNodes = n^3, nnz = N + 2(3mn^2 + 3nm^2 + m^3), m = n-1. */
+int mf_cube_sizes(int n, int *nbElem, int *nbNodes, int64_t *nnz);
+
+/* shuffleSeed == 0 keeps the natural node order. */
+int mf_generate_cube(int n, double jitter, uint64_t geomSeed, uint64_t shuffleSeed,
+                     double *coord, int *elemToNode, int *boundNodesCode);
+
+/* lap: coef[e] = lognormal(0, 0.5). ela: E = lognormal(0, 0.5), nu = U[0.2, 0.35],
+ * coef[2e] = lambda = E nu / ((1+nu)(1-2nu)), coef[2e+1] = mu = E / (2(1+nu)). */
+int mf_generate_coef(int operatorID, int nbElem, uint64_t seed, double *coef);
+
+/* ---------------------------------------------------------------------------------
+ * Host setup (bit-exact twins of the reference's setup, src/main.cc:209-351).
+ * ------------------------------------------------------------------------------- */
+
+/* DC_create_nodeToElem (external DC-lib, called at src/main.cc:247): for each node
+ * the 0-based ids of its elements in ascending order. index[N+1], value[4nE]. */
+int mf_create_nodeToElem(const int *elemToNode, int nbElem, int nbNodes, int *index,
+                         int *value);
+
+/* create_nodeToNode (src/matrix.cc:55-91): row[N+1], col[nbEdges]; first-seen column
+ * order, diagonal included. Returns the nnz (>= 0) or MF_ERR_NNZ when the pattern
+ * does not fit in nbEdges (the reference overflows silently, src/matrix.cc:81). */
+int mf_create_nodeToNode(int *row, int *col, const int *n2eIndex, const int *n2eValue,
+                         const int *elemToNode, int nbNodes, int nbEdges);
+
+/* create_elemToEdge (src/matrix.cc:25-52): elemToEdge[16 nE] CSR positions. */
+int mf_create_elemToEdge(const int *row, const int *col, const int *elemToNode,
+                         int *elemToEdge, int nbElem);
+
+/* coloring_creation (src/coloring.cc:84-109): greedy first-fit over elements sharing
+ * a node; colorPerm[old] = new position (stable sort by color); colorToElem has
+ * MF_MAX_COLOR+1 slots (the reference's global colorToElem, src/coloring.cc:103);
+ * colorPart[nE] (may be NULL) receives each element's color. */
+int mf_coloring_creation(const int *elemToNode, int *colorPart, int *colorPerm,
+                         int nbElem, int nbNodes, int *colorToElem, int *nbColors);
+
+/* DC_permute_int_2d_array (src/main.cc:229): row old moves to row perm[old]. */
+int mf_permute_int_2d(int *tab, const int *perm, int n, int dim);
+int mf_permute_double_2d(double *tab, const int *perm, int n, int dim);
+
+/* dqmrd4_ + e_essbcm_ (src/main.cc:340-345; src/Fortran/qdmrd4.f,
+ * src/Fortran/e_cgmelissa.F): checkBounds[3N] from boundNodesCode[N]. nbBoundNodes
+ * is the header's count (the capacity DQMRD4 is given). */
+int mf_boundary_mask(const int *boundNodesCode, int nbNodes, int nbBoundNodes,
+                     int *checkBounds);
+
+/* compute_double_norm (src/FEM.cc:48-56), host, serial order. */
+double mf_compute_double_norm(const double *tab, int64_t size);
+
+/* ---------------------------------------------------------------------------------
+ * GPU: drop-in twins of the per-iteration entry points (device pointers).
+ * ------------------------------------------------------------------------------- */
+
+/* Number of visible GPUs (>= 0), or MF_ERR_HIP. */
+int mf_device_count(void);
+
+/* Make `device` current for this thread and check it is gfx950. */
+int mf_device_init(int device);
+
+/* Device memory, streams and events through libminifem's own HIP runtime (no
+ * reference counterpart: the reference is host-only). Copies are synchronous with
+ * respect to the host except mf_memcpy_d2d, which is only enqueued. */
+int mf_dev_alloc(void **ptr, size_t bytes);
+int mf_dev_free(void *ptr);
+int mf_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int mf_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int mf_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+int mf_stream_create(void **stream);
+int mf_stream_destroy(void *stream);
+int mf_stream_sync(void *stream);
+int mf_device_sync(void);
+int mf_event_create(void **event);
+int mf_event_destroy(void *event);
+int mf_event_record(void *event, void *stream);
+int mf_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* assembly (src/headers/assembly.h:61-74, called at src/FEM.cc:183), COLORING
+ * semantics (src/assembly.cc:593-611, :660-673): zero values[nbEdges*operatorDim],
+ * then one race-free launch per color over elements [colorToElem[c], colorToElem[c+1])
+ * of the color-permuted elemToNode. elemToEdge (device, 16 per element) gives the
+ * scatter positions; NULL selects the reference's default row search
+ * (src/assembly.cc:415-421). colorToElem is a HOST array of nbColors+1 offsets (the
+ * reference's global). Device: coord, values, row, col, elemToNode, elemToEdge. */
+int mf_assembly(const double *coord, double *values, const int *row, const int *col,
+                const int *elemToNode, const int *elemToEdge, int nbElem, int nbEdges,
+                int operatorDim, int operatorID, const int *colorToElem, int nbColors,
+                void *stream);
+
+/* mf_assembly with a per-element coefficient (device; lap: coef[nE]; ela: lambda,mu
+ * pairs coef[2nE]; NULL = the reference's constants) and precision MF_PREC_*. */
+int mf_assembly_ex(const double *coord, const double *coef, double *values,
+                   const int *row, const int *col, const int *elemToNode,
+                   const int *elemToEdge, int nbElem, int nbEdges, int operatorDim,
+                   int operatorID, const int *colorToElem, int nbColors, int precision,
+                   void *stream);
+
+/* prec_init (src/preconditioner.cc:52-87, called at src/FEM.cc:199). Device. */
+int mf_prec_init(double *prec, const double *values, const int *row, const int *col,
+                 int nbNodes, int operatorDim, void *stream);
+
+/* prec_inversion (src/preconditioner.cc:25-49, called at src/FEM.cc:231) with the
+ * per-node ela_invert_prec of src/Fortran/elasclpr.f:2-56 (Dirichlet mask + 3x3
+ * DGETRF/DGETRI). singular (device int, may be NULL) is incremented once per singular
+ * ela block (the reference only prints LAPACK's info). Device pointers. */
+int mf_prec_inversion(double *prec, const int *row, const int *col,
+                      const int *checkBounds, int nbNodes, int operatorID,
+                      int *singular, void *stream);
+
+/* compute_double_norm (src/FEM.cc:48-56) on the device: deterministic tree sum of
+ * squares, sqrt on the host. x is device memory; blocks until the result is ready. */
+int mf_norm(const double *x, int64_t n, double *result, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,108 @@
+// Device memory, streams and events for callers that have no GPU runtime of their own
+// (the Python binding, the batched runner and the serving loop). Everything goes
+// through the one HIP runtime libminifem is linked against.
+#include <hip/hip_runtime.h>
+
+#include "minifem.h"
+
+extern "C" int mf_dev_alloc(void **ptr, size_t bytes)
+{
+    if (!ptr) return MF_ERR_ARG;
+    *ptr = nullptr;
+    if (bytes == 0) bytes = 1;
+    return hipMalloc(ptr, bytes) == hipSuccess ? MF_OK : MF_ERR_NOMEM;
+}
+
+extern "C" int mf_dev_free(void *ptr)
+{
+    if (!ptr) return MF_OK;
+    return hipFree(ptr) == hipSuccess ? MF_OK : MF_ERR_HIP;
+}
+
+extern "C" int mf_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (bytes == 0) return MF_OK;
+    if (!dst || !src) return MF_ERR_ARG;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) != hipSuccess) return MF_ERR_HIP;
+    return hipStreamSynchronize(s) == hipSuccess ? MF_OK : MF_ERR_HIP;
+}
+
+extern "C" int mf_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (bytes == 0) return MF_OK;
+    if (!dst || !src) return MF_ERR_ARG;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s) != hipSuccess) return MF_ERR_HIP;
+    return hipStreamSynchronize(s) == hipSuccess ? MF_OK : MF_ERR_HIP;
+}
+
+extern "C" int mf_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (bytes == 0) return MF_OK;
+    if (!dst || !src) return MF_ERR_ARG;
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice,
+                          reinterpret_cast<hipStream_t>(stream)) == hipSuccess
+               ? MF_OK
+               : MF_ERR_HIP;
+}
+
+extern "C" int mf_stream_create(void **stream)
+{
+    if (!stream) return MF_ERR_ARG;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return MF_ERR_HIP;
+    *stream = s;
+    return MF_OK;
+}
+
+extern "C" int mf_stream_destroy(void *stream)
+{
+    if (!stream) return MF_OK;
+    return hipStreamDestroy(reinterpret_cast<hipStream_t>(stream)) == hipSuccess ? MF_OK : MF_ERR_HIP;
+}
+
+extern "C" int mf_stream_sync(void *stream)
+{
+    return hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)) == hipSuccess ? MF_OK
+                                                                                      : MF_ERR_HIP;
+}
+
+extern "C" int mf_device_sync(void)
+{
+    return hipDeviceSynchronize() == hipSuccess ? MF_OK : MF_ERR_HIP;
+}
+
+extern "C" int mf_event_create(void **event)
+{
+    if (!event) return MF_ERR_ARG;
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return MF_ERR_HIP;
+    *event = e;
+    return MF_OK;
+}
+
+extern "C" int mf_event_destroy(void *event)
+{
+    if (!event) return MF_OK;
+    return hipEventDestroy(reinterpret_cast<hipEvent_t>(event)) == hipSuccess ? MF_OK : MF_ERR_HIP;
+}
+
+extern "C" int mf_event_record(void *event, void *stream)
+{
+    if (!event) return MF_ERR_ARG;
+    return hipEventRecord(reinterpret_cast<hipEvent_t>(event),
+                          reinterpret_cast<hipStream_t>(stream)) == hipSuccess
+               ? MF_OK
+               : MF_ERR_HIP;
+}
+
+extern "C" int mf_event_elapsed_ms(void *start, void *stop, float *ms)
+{
+    if (!start || !stop || !ms) return MF_ERR_ARG;
+    if (hipEventSynchronize(reinterpret_cast<hipEvent_t>(stop)) != hipSuccess) return MF_ERR_HIP;
+    return hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start),
+                               reinterpret_cast<hipEvent_t>(stop)) == hipSuccess
+               ? MF_OK
+               : MF_ERR_HIP;
+}

@@ -1,0 +1,196 @@
+"""GPU entry points over the C ABI.
+
+Device memory, streams and events come from libminifem itself (mf_dev_alloc, ...), so
+the process holds exactly one HIP runtime: the one libminifem is linked against. Do not
+initialise another GPU runtime (e.g. torch.cuda) in the same process.
+
+Mirrors the reference's per-iteration interface: assembly
+(src/headers/assembly.h:61-74), prec_init / prec_inversion
+(src/headers/preconditioner.h:27-32) and compute_double_norm (src/headers/FEM.h:26).
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import OP_ELA, PREC_FP64, check, lib
+
+_dev_ready = False
+
+
+def device_count():
+    n = lib().mf_device_count()
+    return max(n, 0)
+
+
+def init(device=0):
+    global _dev_ready
+    check(lib().mf_device_init(int(device)), "mf_device_init")
+    _dev_ready = True
+
+
+def _ensure():
+    if not _dev_ready:
+        init(0)
+
+
+class DeviceArray:
+    """A flat device buffer of one numpy dtype, freed when garbage collected."""
+
+    def __init__(self, n, dtype):
+        _ensure()
+        self.dtype = np.dtype(dtype)
+        self.size = int(n)
+        self.nbytes = self.size * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(lib().mf_dev_alloc(ctypes.byref(p), self.nbytes), "mf_dev_alloc")
+        self.ptr = p.value
+        self._lib = lib()
+
+    @classmethod
+    def from_numpy(cls, a, dtype=None, stream=None):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(a.size, a.dtype)
+        check(lib().mf_memcpy_h2d(d.ptr, a.ctypes.data, a.nbytes, _sp(stream)), "h2d")
+        return d
+
+    @classmethod
+    def zeros(cls, n, dtype, stream=None):
+        return cls.from_numpy(np.zeros(int(n), dtype), stream=stream)
+
+    def numpy(self, stream=None):
+        out = np.empty(self.size, self.dtype)
+        check(lib().mf_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes, _sp(stream)), "d2h")
+        return out
+
+    def copy_from(self, a, stream=None):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.size
+        check(lib().mf_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes, _sp(stream)), "h2d")
+
+    def fill(self, value, stream=None):
+        self.copy_from(np.full(self.size, value, self.dtype), stream)
+
+    def __del__(self):
+        p = getattr(self, "ptr", None)
+        if p:
+            self._lib.mf_dev_free(p)
+            self.ptr = None
+
+
+def _sp(stream):
+    if stream is None:
+        return None
+    return stream.ptr if isinstance(stream, Stream) else stream
+
+
+def _p(a):
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    raise TypeError(f"expected a DeviceArray, got {type(a)}")
+
+
+class Stream:
+    def __init__(self):
+        _ensure()
+        p = ctypes.c_void_p()
+        check(lib().mf_stream_create(ctypes.byref(p)), "mf_stream_create")
+        self.ptr = p.value
+        self._lib = lib()
+
+    def sync(self):
+        check(self._lib.mf_stream_sync(self.ptr), "mf_stream_sync")
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self._lib.mf_stream_destroy(self.ptr)
+            self.ptr = None
+
+
+class Event:
+    def __init__(self):
+        _ensure()
+        p = ctypes.c_void_p()
+        check(lib().mf_event_create(ctypes.byref(p)), "mf_event_create")
+        self.ptr = p.value
+        self._lib = lib()
+
+    def record(self, stream=None):
+        check(self._lib.mf_event_record(self.ptr, _sp(stream)), "mf_event_record")
+
+    def elapsed_ms(self, end):
+        ms = ctypes.c_float()
+        check(self._lib.mf_event_elapsed_ms(self.ptr, end.ptr, ctypes.byref(ms)), "elapsed")
+        return float(ms.value)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self._lib.mf_event_destroy(self.ptr)
+            self.ptr = None
+
+
+def synchronize():
+    check(lib().mf_device_sync(), "mf_device_sync")
+
+
+def assembly(coord, values, row, col, elemToNode, elemToEdge, nbElem, nbEdges, operatorDim,
+             operatorID, colorToElem, coef=None, precision=PREC_FP64, stream=None):
+    """assembly(...) of the COLORING build on the GPU; colorToElem is a host array."""
+    _ensure()
+    c2e = np.ascontiguousarray(colorToElem, dtype=np.int32)
+    check(lib().mf_assembly_ex(_p(coord), _p(coef), _p(values), _p(row), _p(col), _p(elemToNode),
+                               _p(elemToEdge), int(nbElem), int(nbEdges), int(operatorDim),
+                               int(operatorID), c2e.ctypes.data, len(c2e) - 1, int(precision),
+                               _sp(stream)), "mf_assembly_ex")
+
+
+def prec_init(prec, values, row, col, nbNodes, operatorDim, stream=None):
+    _ensure()
+    check(lib().mf_prec_init(_p(prec), _p(values), _p(row), _p(col), int(nbNodes),
+                             int(operatorDim), _sp(stream)), "mf_prec_init")
+
+
+def prec_inversion(prec, row, col, checkBounds, nbNodes, operatorID, singular=None, stream=None):
+    _ensure()
+    check(lib().mf_prec_inversion(_p(prec), _p(row), _p(col), _p(checkBounds), int(nbNodes),
+                                  int(operatorID), _p(singular), _sp(stream)),
+          "mf_prec_inversion")
+
+
+def norm(x, stream=None):
+    _ensure()
+    out = np.zeros(1)
+    check(lib().mf_norm(_p(x), x.size, out.ctypes.data, _sp(stream)), "mf_norm")
+    return float(out[0])
+
+
+class DeviceMesh:
+    """A Mesh's arrays resident on the GPU, with value/prec buffers."""
+
+    def __init__(self, mesh, with_elem_to_edge=False):
+        _ensure()
+        self.mesh = mesh
+        up = DeviceArray.from_numpy
+        self.coord = up(mesh.coord)
+        self.elemToNode = up(mesh.elemToNode)
+        self.row = up(mesh.row)
+        self.col = up(mesh.col)
+        self.checkBounds = up(mesh.checkBounds)
+        self.coef = None if mesh.coef is None else up(mesh.coef)
+        self.elemToEdge = None
+        if with_elem_to_edge:
+            from .setup import create_elemToEdge
+            self.elemToEdge = up(create_elemToEdge(mesh.row, mesh.col, mesh.elemToNode))
+        self.values = DeviceArray(mesh.nnz * mesh.dim, np.float64)
+        self.prec = DeviceArray(mesh.nbNodes * mesh.dim, np.float64)
+
+    def iterate_colored(self, precision=PREC_FP64, use_coef=True, stream=None):
+        """One FEM iteration (src/FEM.cc:183-232) with the drop-in colored kernels."""
+        m = self.mesh
+        assembly(self.coord, self.values, self.row, self.col, self.elemToNode, self.elemToEdge,
+                 m.nbElem, m.nnz, m.dim, m.operatorID, m.colorToElem,
+                 coef=self.coef if use_coef else None, precision=precision, stream=stream)
+        prec_init(self.prec, self.values, self.row, self.col, m.nbNodes, m.dim, stream=stream)
+        prec_inversion(self.prec, self.row, self.col, self.checkBounds, m.nbNodes, m.operatorID,
+                       stream=stream)
