@@ -9,7 +9,7 @@
  *
  * Host setup functions run on the CPU (they are the reference's one-time setup).
  * Per-iteration functions run on the GPU: pointer arguments documented "device" must
- * be device memory (hipMalloc or a torch cuda tensor); `stream` is a hipStream_t
+ * be device memory (hipMalloc or mf_dev_alloc); `stream` is a hipStream_t
  * passed as void* (NULL = the default stream). Device entry points return after
  * enqueueing work unless stated otherwise.
  */
@@ -155,6 +155,10 @@ int mf_event_destroy(void *event);
 int mf_event_record(void *event, void *stream);
 int mf_event_elapsed_ms(void *start, void *stop, float *ms);
 
+/* Plain streaming device-to-device copy kernel (16-byte aligned, size a multiple of
+ * 16): the HBM bandwidth reference for rooflines and byte-counter calibration. */
+int mf_hbm_copy(void *dst, const void *src, size_t bytes, void *stream);
+
 /* assembly (src/headers/assembly.h:61-74, called at src/FEM.cc:183), COLORING
  * semantics (src/assembly.cc:593-611, :660-673): zero values[nbEdges*operatorDim],
  * then one race-free launch per color over elements [colorToElem[c], colorToElem[c+1])
@@ -190,6 +194,53 @@ int mf_prec_inversion(double *prec, const int *row, const int *col,
 /* compute_double_norm (src/FEM.cc:48-56) on the device: deterministic tree sum of
  * squares, sqrt on the host. x is device memory; blocks until the result is ready. */
 int mf_norm(const double *x, int64_t n, double *result, void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * Plans: a mesh resident on the GPU with a precomputed tiled gather schedule.
+ * One plan = one mesh topology (color-permuted elemToNode, CSR pattern, boundary
+ * mask). mf_plan_assemble runs the whole per-iteration step (src/FEM.cc:183-232):
+ * assembly into values + the inverted block-Jacobi prec, in one fused launch, with the
+ * same per-entry summation order as the COLORING build (ascending permuted element).
+ * ------------------------------------------------------------------------------- */
+
+typedef struct mf_plan mf_plan;
+
+/* Tuning knobs of the tiled schedule (0 = default). */
+typedef struct mf_plan_params {
+    int tile_lds_bytes;   /* LDS budget of one tile's element stage */
+    int block_threads;    /* threads per workgroup (multiple of 64) */
+    int max_tile_rows;    /* cap on rows per tile */
+} mf_plan_params;
+
+/* Host arrays: elemToNode (color-permuted, 1-based, 4nE), row[N+1], col[nnz],
+ * checkBounds[3N] (may be NULL), coord[3N] (may be NULL; used only to group nodes into
+ * tiles by coordinate bisection, else a breadth-first node order is used). The plan
+ * copies what it needs to the GPU and checks every index its kernel follows. Limits:
+ * rows of at most 256 entries, at most 4095 elements incident to one tile. */
+int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nbNodes, int nnz,
+                   const int *elemToNode, const int *row, const int *col,
+                   const int *checkBounds, const double *coord,
+                   const mf_plan_params *params);
+int mf_plan_destroy(mf_plan *plan);
+
+/* Builds and validates the schedule on the host only (no GPU needed) and returns the
+ * statistics mf_plan_stats would report. For tests and the autotuner. */
+int mf_plan_host_stats(int operatorID, int nbElem, int nbNodes, int nnz,
+                       const int *elemToNode, const int *row, const int *col,
+                       const double *coord, const mf_plan_params *params, int64_t out[8]);
+
+/* Plan statistics: out[0] tiles, [1] tile-element slots, [2] tasks, [3]
+ * contributions, [4] metadata bytes on the device, [5] LDS bytes per workgroup,
+ * [6] threads per workgroup, [7] rows per tile (max). */
+int mf_plan_stats(const mf_plan *plan, int64_t out[8]);
+
+/* One iteration: values (assembly, src/FEM.cc:183) and the inverted block-Jacobi prec
+ * (prec_init + prec_inversion, src/FEM.cc:199-232) in one launch. Device: coord[3N],
+ * coef (NULL = reference constants), values[nnz*d], prec[N*d], singular (may be NULL;
+ * incremented per singular ela block). */
+int mf_plan_assemble(mf_plan *plan, const double *coord, const double *coef,
+                     double *values, double *prec, int precision, int *singular,
+                     void *stream);
 
 #ifdef __cplusplus
 }

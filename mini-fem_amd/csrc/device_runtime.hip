@@ -3,7 +3,38 @@
 // through the one HIP runtime libminifem is linked against.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdint>
+
 #include "minifem.h"
+
+namespace {
+
+// Streaming copy, 16 B per lane per iteration: the HBM bandwidth reference the
+// roofline's measured fraction and the byte counters are calibrated against.
+__global__ void __launch_bounds__(256)
+    k_hbm_copy(uint4 *__restrict__ dst, const uint4 *__restrict__ src, int64_t n16)
+{
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+
+}  // namespace
+
+extern "C" int mf_hbm_copy(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (bytes == 0) return MF_OK;
+    if (!dst || !src || (bytes & 15) || (reinterpret_cast<uintptr_t>(dst) & 15) ||
+        (reinterpret_cast<uintptr_t>(src) & 15))
+        return MF_ERR_ARG;
+    const int64_t n16 = (int64_t)(bytes / 16);
+    const int64_t blocks = std::min<int64_t>((n16 + 255) / 256, 256 * 32);
+    hipLaunchKernelGGL(k_hbm_copy, dim3((unsigned)blocks), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), reinterpret_cast<uint4 *>(dst),
+                       reinterpret_cast<const uint4 *>(src), n16);
+    return hipGetLastError() == hipSuccess ? MF_OK : MF_ERR_HIP;
+}
 
 extern "C" int mf_dev_alloc(void **ptr, size_t bytes)
 {

@@ -71,7 +71,18 @@ SIGNATURES = {
     "mf_event_destroy": (_i, [_p]),
     "mf_event_record": (_i, [_p, _p]),
     "mf_event_elapsed_ms": (_i, [_p, _p, _p]),
+    "mf_hbm_copy": (_i, [_p, _p, ctypes.c_size_t, _p]),
+    "mf_plan_create": (_i, [_p, _i, _i, _i, _i, _p, _p, _p, _p, _p, _p]),
+    "mf_plan_destroy": (_i, [_p]),
+    "mf_plan_stats": (_i, [_p, _p]),
+    "mf_plan_assemble": (_i, [_p, _p, _p, _p, _p, _i, _p, _p]),
+    "mf_plan_host_stats": (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p]),
 }
+
+
+class PlanParams(ctypes.Structure):
+    _fields_ = [("tile_lds_bytes", ctypes.c_int), ("block_threads", ctypes.c_int),
+                ("max_tile_rows", ctypes.c_int)]
 
 
 class MiniFemError(RuntimeError):

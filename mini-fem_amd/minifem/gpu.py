@@ -194,3 +194,41 @@ class DeviceMesh:
         prec_init(self.prec, self.values, self.row, self.col, m.nbNodes, m.dim, stream=stream)
         prec_inversion(self.prec, self.row, self.col, self.checkBounds, m.nbNodes, m.operatorID,
                        stream=stream)
+
+
+class Plan:
+    """A mesh topology resident on the GPU with its tiled gather schedule
+    (mf_plan_create). assemble() runs one fused iteration: values + inverted prec."""
+
+    def __init__(self, mesh, tile_lds_bytes=0, block_threads=0, max_tile_rows=0,
+                 use_coord=True):
+        from ._lib import PlanParams
+        _ensure()
+        self._lib = lib()
+        self.mesh = mesh
+        params = PlanParams(int(tile_lds_bytes), int(block_threads), int(max_tile_rows))
+        p = ctypes.c_void_p()
+        check(self._lib.mf_plan_create(
+            ctypes.byref(p), mesh.operatorID, mesh.nbElem, mesh.nbNodes, mesh.nnz,
+            mesh.elemToNode.ctypes.data, mesh.row.ctypes.data, mesh.col.ctypes.data,
+            mesh.checkBounds.ctypes.data, mesh.coord.ctypes.data if use_coord else None,
+            ctypes.byref(params)), "mf_plan_create")
+        self.ptr = p.value
+
+    def stats(self):
+        out = np.zeros(8, np.int64)
+        check(self._lib.mf_plan_stats(self.ptr, out.ctypes.data), "mf_plan_stats")
+        keys = ["tiles", "tile_elems", "tasks", "contribs", "meta_bytes", "lds_bytes",
+                "block_threads", "max_rows"]
+        return dict(zip(keys, (int(x) for x in out)))
+
+    def assemble(self, coord, coef, values, prec, precision=PREC_FP64, singular=None,
+                 stream=None):
+        check(self._lib.mf_plan_assemble(self.ptr, _p(coord), _p(coef), _p(values), _p(prec),
+                                         int(precision), _p(singular), _sp(stream)),
+              "mf_plan_assemble")
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self._lib.mf_plan_destroy(self.ptr)
+            self.ptr = None

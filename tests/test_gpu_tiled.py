@@ -1,0 +1,152 @@
+"""GPU parity of the fused tiled plan (mf_plan_create / mf_plan_assemble) against the
+CPU oracle, through the C ABI.
+
+Bar: values and prec bit-exact against the oracle's COLORING-order restatement for
+FP64; mixed precision bit-exact against the oracle's mixed restatement and within
+1e-6 relative Frobenius error of FP64. At the 100^3 size the oracle is not run;
+size-independent properties are checked instead (exact block symmetry, vanishing row
+sums, prec times the masked diagonal block = I, agreement with the drop-in colored
+path bit for bit).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def build(mf, orc, n, op, jitter=0.2, seed=1, shuffle=0, coef_seed=None, oracle=True):
+    coord, e2n, code, nnz = mf.generate_cube(n, jitter, seed, shuffle)
+    coef = None if coef_seed is None else mf.generate_coef(op, e2n.size // 4, coef_seed)
+    mesh = mf.Mesh(coord, e2n, code, nnz, op, coef=coef)
+    om = orc.OracleMesh(coord, e2n, code, nnz, op, coef=coef) if oracle else None
+    return mesh, om
+
+
+def run_plan(gpu, mesh, precision=0, **params):
+    plan = gpu.Plan(mesh, **params)
+    dm = gpu.DeviceMesh(mesh)
+    plan.assemble(dm.coord, dm.coef, dm.values, dm.prec, precision=precision)
+    return plan, dm
+
+
+@pytest.mark.parametrize("op", [0, 1])
+@pytest.mark.parametrize("coef_seed", [None, 101])
+def test_tiled_bit_exact_30_cube(mf, orc, gpu, op, coef_seed):
+    print(f"[tiled] op={op} coef={coef_seed}", flush=True)
+    mesh, om = build(mf, orc, 30, op, seed=1 + op, coef_seed=coef_seed)
+    plan, dm = run_plan(gpu, mesh)
+    ov = om.assemble()
+    assert np.array_equal(dm.values.numpy(), ov)
+    assert np.array_equal(dm.prec.numpy(), om.prec(ov)[0])
+
+
+@pytest.mark.parametrize("op", [0, 1])
+@pytest.mark.parametrize("params", [
+    dict(block_threads=128), dict(block_threads=512), dict(tile_lds_bytes=8192),
+    dict(tile_lds_bytes=150 * 1024, max_tile_rows=254), dict(use_coord=False),
+    dict(max_tile_rows=1),
+])
+def test_tiled_schedule_variants(mf, orc, gpu, op, params):
+    print(f"[tiled variant] op={op} {params}", flush=True)
+    mesh, om = build(mf, orc, 12, op, jitter=0.25, seed=3, shuffle=203, coef_seed=9)
+    plan, dm = run_plan(gpu, mesh, **params)
+    ov = om.assemble()
+    assert np.array_equal(dm.values.numpy(), ov)
+    assert np.array_equal(dm.prec.numpy(), om.prec(ov)[0])
+
+
+@pytest.mark.parametrize("op", [0, 1])
+def test_tiled_mixed(mf, orc, gpu, op):
+    print(f"[tiled mixed] op={op}", flush=True)
+    mesh, om = build(mf, orc, 20, op, coef_seed=55)
+    plan, dm = run_plan(gpu, mesh, precision=mf.PREC_MIXED)
+    vals = dm.values.numpy()
+    assert np.array_equal(vals, om.assemble(mixed=True))
+    ov = om.assemble()
+    rel = np.linalg.norm(vals - ov) / np.linalg.norm(ov)
+    print(f"tiled mixed vs fp64 relative Frobenius error: {rel:.3e}")
+    assert rel <= 1e-6
+    assert np.array_equal(dm.prec.numpy(), om.prec(vals)[0])
+
+
+@pytest.mark.parametrize("op", [0, 1])
+def test_tiled_tiny_and_isolated(mf, orc, gpu, op):
+    # single tet, two tets, and a mesh with an unused node (empty CSR row)
+    cases = [
+        (np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, 0]], np.float64).ravel(),
+         np.array([1, 2, 3, 4], np.int32), np.zeros(4, np.int32), 16),
+        (np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 1]], np.float64).ravel(),
+         np.array([1, 2, 3, 4, 2, 3, 4, 5], np.int32), np.array([52, 53, 54, 10, 0], np.int32), 23),
+        (np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [5, 5, 5]], np.float64).ravel(),
+         np.array([1, 2, 3, 4], np.int32), np.array([0, 0, 0, 0, 53], np.int32), 16),
+    ]
+    for coord, e2n, code, nnz in cases:
+        mesh = mf.Mesh(coord, e2n, code, nnz, op)
+        om = orc.OracleMesh(coord, e2n, code, nnz, op)
+        plan, dm = run_plan(gpu, mesh)
+        ov = om.assemble()
+        assert np.array_equal(dm.values.numpy(), ov)
+        p_gpu, p_ref = dm.prec.numpy(), om.prec(ov)[0]
+        assert np.array_equal(p_gpu, p_ref) or (
+            np.array_equal(np.isinf(p_gpu), np.isinf(p_ref)) and
+            np.array_equal(p_gpu[~np.isinf(p_gpu)], p_ref[~np.isinf(p_ref)]))
+
+
+def test_tiled_equals_colored_and_deterministic(mf, orc, gpu):
+    mesh, _ = build(mf, orc, 24, 1, jitter=0.25, seed=3, shuffle=203, coef_seed=103, oracle=False)
+    plan, dm = run_plan(gpu, mesh)
+    v1, p1 = dm.values.numpy(), dm.prec.numpy()
+    plan.assemble(dm.coord, dm.coef, dm.values, dm.prec)
+    assert np.array_equal(v1, dm.values.numpy()) and np.array_equal(p1, dm.prec.numpy())
+    dm.iterate_colored()
+    assert np.array_equal(v1, dm.values.numpy()) and np.array_equal(p1, dm.prec.numpy())
+
+
+@pytest.mark.parametrize("op", [0, 1])
+def test_tiled_large_properties(mf, orc, gpu, op):
+    """100^3 shuffled jittered cube (the benchmark workload): properties that do not need
+    the oracle at full size."""
+    print(f"[tiled 100^3] op={op}", flush=True)
+    mesh, _ = build(mf, orc, 100, op, jitter=0.25, seed=3, shuffle=203, coef_seed=103,
+                    oracle=False)
+    plan, dm = run_plan(gpu, mesh)
+    vals, prec = dm.values.numpy(), dm.prec.numpy()
+    assert np.all(np.isfinite(vals)) and np.all(np.isfinite(prec))
+    d = mesh.dim
+    b = 3 if op else 1
+    n = mesh.nbNodes
+    rows = np.repeat(np.arange(n), np.diff(mesh.row))
+    cols = mesh.col - 1
+    V = vals.reshape(-1, b, b)
+    # exact symmetry: block(i,j) == block(j,i)^T bit for bit
+    key = rows.astype(np.int64) * n + cols
+    tkey = cols.astype(np.int64) * n + rows
+    order = np.argsort(key)
+    pos_t = order[np.searchsorted(key[order], tkey)]
+    assert np.array_equal(V, V[pos_t].transpose(0, 2, 1))
+    # row sums of every block row vanish (constants are in the kernel of the operator)
+    rs = np.zeros((n, b, b))
+    np.add.at(rs, rows, V)
+    scale = np.abs(V).max()
+    assert np.abs(rs).max() < 1e-9 * scale
+    # prec inverts the (masked) diagonal block
+    diag = np.nonzero(rows == cols)[0]
+    assert diag.size == n
+    Dg = V[diag]
+    P = prec.reshape(n, b, b)
+    if op == 0:
+        assert np.array_equal(P[:, 0, 0], 1.0 / Dg[:, 0, 0])
+    else:
+        cb = mesh.checkBounds.reshape(3, n).T.astype(bool)
+        A = Dg.transpose(0, 2, 1).copy()  # Fortran view of each block
+        for k in range(3):
+            A[cb[:, k], k, :] = 0
+            A[cb[:, k], :, k] = 0
+            A[cb[:, k], k, k] = 1
+        Pf = P.transpose(0, 2, 1)
+        err = np.abs(np.einsum("nij,njk->nik", Pf, A) - np.eye(3)).max()
+        assert err < 1e-10, err
+    # the drop-in colored path gives the same bits
+    dm.iterate_colored()
+    assert np.array_equal(vals, dm.values.numpy())
+    assert np.array_equal(prec, dm.prec.numpy())
