@@ -16,26 +16,26 @@
 
 struct mf_plan {
     int op = 0, nE = 0, N = 0, nnz = 0, dim = 1;
-    int ntiles = 0, block = 256;
+    int ntiles = 0, block = 256, mask_words = 1;
     int lds_fp64 = 0, lds_mixed = 0;
     int n_isolated = 0;
     int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     mf::TileHdr *tiles = nullptr;
     int32_t *tile_elems = nullptr;
-    uchar4 *tile_verts = nullptr;
+    uint32_t *tile_verts = nullptr;
     int32_t *tile_nodes = nullptr;
-    mf::RowInfo *tile_rowinfo = nullptr;
-    mf::ChunkHdr *chunks = nullptr;
-    uint32_t *tasks = nullptr;
-    uint16_t *contribs = nullptr;
+    int4 *tile_rowinfo = nullptr;
+    uint16_t *inc = nullptr;
+    uint32_t *ent_mask = nullptr;
+    uint16_t *ent_info = nullptr;
     int32_t *cb = nullptr;
     int32_t *isolated = nullptr;
 };
 
 namespace {
 
-constexpr int kXcd = 8;      // MI355X: workgroups are dealt to the 8 XCDs round-robin
-constexpr int kDiagPre = 8;  // diagonal-chunk groups (32 contributions) loaded at once
+constexpr int kXcd = 8;  // MI355X: workgroups are dealt to the 8 XCDs round-robin
+constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
 
 template <typename T>
 struct Vec2;
@@ -48,59 +48,64 @@ struct Vec2<float> {
     using type = float2;
 };
 
-// Sum of one lane's contributions for the G entries packed in `pk` (G = 2 or 4).
-template <int OP, typename T, typename T2, int S, int G>
-__device__ __forceinline__ void accumulate(const T *__restrict__ stage, const uint32_t (&pk)[G / 2],
-                                           double (&acc)[OP == MF_OP_ELA ? 9 : 1])
+template <int MW>
+struct MaskT;
+template <>
+struct MaskT<1> {
+    using type = uint32_t;
+};
+template <>
+struct MaskT<2> {
+    using type = uint64_t;
+};
+
+template <typename M>
+__device__ __forceinline__ int lowest_bit(M m)
 {
-#pragma unroll
-    for (int j = 0; j < G; j++) {
-        const uint32_t en = (pk[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
-        const T *sl = stage + __umul24(en >> 4, S);
-        const int a = (en >> 2) & 3, k = en & 3;
-        const T2 axy = *reinterpret_cast<const T2 *>(sl + 2 * a);
-        const T2 kxy = *reinterpret_cast<const T2 *>(sl + 2 * k);
-        const T ga[3] = {axy.x, axy.y, sl[8 + a]};
-        const T gk[3] = {kxy.x, kxy.y, sl[8 + k]};
-        if constexpr (OP == MF_OP_ELA) {
-            const T2 c01 = *reinterpret_cast<const T2 *>(sl + 12);
-            T b[9];
-            mf::ela_block(ga, gk, c01.x, c01.y, sl[14], b);
-#pragma unroll
-            for (int q = 0; q < 9; q++) acc[q] += (double)b[q];
-        } else {
-            const T x = sl[12] * mf::lap_entry(ga, gk);
-            acc[0] += (double)x;
-        }
-    }
+    if constexpr (sizeof(M) == 8) return __builtin_ctzll(m);
+    else return __builtin_ctz(m);
 }
 
-// One workgroup = one tile. blockIdx -> tile is XCD-aware: XCD x (= blockIdx % 8) walks
+// One tile element's gradient of vertex v from its stage slot: {x, y, z}.
+template <typename T, typename T2>
+__device__ __forceinline__ void stage_grad(const T *sl, int v, T (&g)[3])
+{
+    const T2 xy = *reinterpret_cast<const T2 *>(sl + 2 * v);
+    g[0] = xy.x;
+    g[1] = xy.y;
+    g[2] = sl[8 + v];
+}
+
+// One tile is one workgroup. blockIdx -> tile is XCD-aware: XCD x (= blockIdx % 8) walks
 // the x-th contiguous eighth of the tiles in partition order, so tiles that share
 // elements and nodes run on the same XCD close in time and share its L2.
 //
-// Global loads are issued in as few dependent rounds as possible: (1) the tile's ids,
-// vertex lists, row info and the first diagonal chunk; (2) the coordinate and
-// coefficient gathers. gfx950's vmcnt retires loads and stores in order, so a load
-// issued after a store waits for that store: every load of a row chunk is issued before
-// the previous row chunk's stores, and no other store precedes a load.
-template <int OP, int PREC, int BT>
+// Global loads come in two dependent rounds: (1) everything static the tile needs --
+// node and element ids, vertex lists, row info, incidence lists and this wave's entry
+// masks; (2) the coordinate and coefficient gathers. Phase B then reads only LDS and
+// registers, so its stores are never followed by a load (gfx950's vmcnt retires loads
+// and stores in order).
+template <int OP, int PREC, int BT, int MW>
 __global__ void __launch_bounds__(BT)
     k_tiled(const mf::TileHdr *__restrict__ tiles, int ntiles,
-            const int32_t *__restrict__ tile_elems, const uchar4 *__restrict__ tile_verts,
-            const int32_t *__restrict__ tile_nodes, const mf::RowInfo *__restrict__ rowinfo,
-            const mf::ChunkHdr *__restrict__ chunks, const uint32_t *__restrict__ tasks,
-            const uint16_t *__restrict__ contribs, const double *__restrict__ coord,
+            const int32_t *__restrict__ tile_elems, const uint32_t *__restrict__ tile_verts,
+            const int32_t *__restrict__ tile_nodes, const int4 *__restrict__ rowinfo,
+            const uint4 *__restrict__ inc8, const uint32_t *__restrict__ ent_mask,
+            const uint16_t *__restrict__ ent_info, const double *__restrict__ coord,
             const double *__restrict__ coef, double *__restrict__ values,
             double *__restrict__ prec, int *__restrict__ singular)
 {
     using T = typename std::conditional<PREC == MF_PREC_FP64, double, float>::type;
     using T2 = typename Vec2<T>::type;
-    constexpr int S = OP == MF_OP_ELA ? mf::kStageEla : mf::kStageLap;
-    constexpr int NA = OP == MF_OP_ELA ? 9 : 1;
-    constexpr int GR = mf::kRowGroup, GD = mf::kDiagGroup, PF = mf::kRowPrefetch;
+    using M = typename MaskT<MW>::type;
+    constexpr bool ELA = OP == MF_OP_ELA;
+    constexpr int S = ELA ? mf::kStageEla : mf::kStageLap;
+    constexpr int NA = ELA ? 9 : 1;
     constexpr int NW = BT / 64;
-    constexpr int EPT = 8;  // tile elements per thread handled in registers (TE <= EPT*BT)
+    constexpr int EPT = mf::kMaxTileElems / BT;                // tile elements per thread
+    constexpr int NPT = (mf::kMaxTileNodes + BT - 1) / BT;     // tile nodes per thread
+    constexpr int IPT = (mf::kMaxTileInc / 8 + BT - 1) / BT;   // 16-byte incidence loads
+    constexpr int PB = mf::kMaxTileBlocks / NW;                // work blocks per wave
     extern __shared__ __align__(16) unsigned char lds_raw[];
 
     const int per = (ntiles + kXcd - 1) / kXcd;
@@ -109,92 +114,135 @@ __global__ void __launch_bounds__(BT)
     if (slot >= per || tile >= ntiles) return;
     const mf::TileHdr h = tiles[tile];
     const int NN = h.n_node, TE = h.n_elem, R = h.n_rows;
-    const mf::TileLds L = mf::tile_lds_layout(NN, TE, R, S * (int)sizeof(T), NA * 8);
-
+    const mf::TileLds L = mf::tile_lds_layout(NN, TE, R, h.n_inc, S * (int)sizeof(T));
     T *stage = reinterpret_cast<T *>(lds_raw + L.stage);
-    double *ldiag = reinterpret_cast<double *>(lds_raw + L.diag);
-    double *lprec = reinterpret_cast<double *>(lds_raw + L.prec);
+    int4 *lri = reinterpret_cast<int4 *>(lds_raw + L.rinfo);
     double *lc = reinterpret_cast<double *>(lds_raw + L.coords);
-    int *lrp = reinterpret_cast<int *>(lds_raw + L.rowptr);
-    int *lrl = reinterpret_cast<int *>(lds_raw + L.rowlen);
-    int *lmask = reinterpret_cast<int *>(lds_raw + L.mask);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int32_t *nodes = tile_nodes + h.node_start;
+    int *lrnode = reinterpret_cast<int *>(lds_raw + L.rnode);
+    uint32_t *lverts = reinterpret_cast<uint32_t *>(lds_raw + L.verts);
+    uint16_t *linc = reinterpret_cast<uint16_t *>(lds_raw + L.inc);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // Work blocks: diagonal block w (rows 64w..) on wave w; entry blocks m (CSR entries
+    // 64m..) on the other waves while they hold at most 4 each (a diagonal block costs
+    // about as much as four entry blocks), else on all waves round-robin.
+    const int nDB = mf::diag_blocks(ELA, R);
+    const int nEB = (h.n_ent + 63) / 64;
+    const bool spread = nEB > 4 * (NW - nDB);
+    const int m0 = spread ? wave : wave - nDB, mstep = spread ? NW : NW - nDB;
 
-    // A, round 1: ids, vertex lists, row info (all independent of each other)
-    int nodeId[(mf::kMaxTileNodes + BT - 1) / BT];
+    // ---- round 1: static tile data
+    int nodeId[NPT];
 #pragma unroll
-    for (int r = 0; r < (mf::kMaxTileNodes + BT - 1) / BT; r++) {
-        const int i = threadIdx.x + r * BT;
-        nodeId[r] = i < NN ? nodes[i] : 0;
+    for (int r = 0; r < NPT; r++) {
+        const int i = tid + r * BT;
+        nodeId[r] = i < NN ? tile_nodes[h.node_start + i] : 0;
     }
     int elemId[EPT];
-    uchar4 vert[EPT];
+    uint32_t vert[EPT];
 #pragma unroll
     for (int r = 0; r < EPT; r++) {
-        const int t = threadIdx.x + r * BT;
+        const int t = tid + r * BT;
         elemId[r] = t < TE ? tile_elems[h.elem_start + t] : 0;
-        vert[r] = t < TE ? tile_verts[h.elem_start + t] : make_uchar4(0, 0, 0, 0);
+        vert[r] = t < TE ? tile_verts[h.elem_start + t] : 0u;
     }
-    for (int i = threadIdx.x; i < R; i += BT) {
-        const mf::RowInfo ri = rowinfo[h.rinfo_start + i];
-        lrp[i] = ri.rowptr;
-        lrl[i] = ri.len_mask & 0xFFFF;
-        lmask[i] = ri.len_mask >> 16;
+    int4 rinf[NPT];
+#pragma unroll
+    for (int r = 0; r < NPT; r++) {
+        const int i = tid + r * BT;
+        rinf[r] = i < R ? rowinfo[h.rinfo_start + i] : make_int4(0, 0, 0, 0);
     }
-    // the first diagonal chunk of this wave, all its groups
-    uint2 dpre[kDiagPre];
-    uint32_t dtask = mf::kNoTask;
-    int dgroups = 0;
-    const uint2 *dcg = nullptr;
-    auto dprefetch = [&](int c) {
-        const mf::ChunkHdr chh = chunks[h.dchunk_start + c];
-        dtask = tasks[(int64_t)(h.dchunk_start + c) * 64 + lane];
-        dcg = reinterpret_cast<const uint2 *>(contribs + chh.contrib_start) + lane;
-        dgroups = (chh.steps + GD - 1) / GD;
+    uint4 incv[IPT];
 #pragma unroll
-        for (int g = 0; g < kDiagPre; g++) dpre[g] = g < dgroups ? dcg[g * 64] : make_uint2(0u, 0u);
-    };
-    if (wave < h.n_dchunks) dprefetch(wave);
-
-    // A, round 2: coordinate and coefficient gathers
-#pragma unroll
-    for (int r = 0; r < (mf::kMaxTileNodes + BT - 1) / BT; r++) {
-        const int i = threadIdx.x + r * BT;
-        if (i < NN) {
-            const double *p = coord + (int64_t)nodeId[r] * 3;
-            const double x = p[0], y = p[1], z = p[2];
-            lc[i * 3 + 0] = x;
-            lc[i * 3 + 1] = y;
-            lc[i * 3 + 2] = z;
-        }
+    for (int r = 0; r < IPT; r++) {
+        const int i = tid + r * BT;
+        incv[r] = i < h.n_inc / 8 ? inc8[h.inc_start / 8 + i] : make_uint4(0, 0, 0, 0);
     }
-    double kc[EPT][3];
+    uint32_t pinf[PB];
+    M pm[PB];
 #pragma unroll
-    for (int r = 0; r < EPT; r++) {
-        const int t = threadIdx.x + r * BT;
-        kc[r][0] = 1.0;
-        kc[r][1] = kc[r][2] = 0.0;
-        if (t < TE) {
-            if (OP == MF_OP_ELA) {
-                const mf::ElaCoef k = mf::ela_coef(coef, elemId[r]);
-                kc[r][0] = k.cd;
-                kc[r][1] = k.co;
-                kc[r][2] = k.cm;
-            } else if (coef) {
-                kc[r][0] = coef[elemId[r]];
+    for (int k = 0; k < PB; k++) {
+        const int idx = (m0 + k * mstep) * 64 + lane;
+        pinf[k] = kNoEntry;
+        pm[k] = 0;
+        if (m0 >= 0 && idx < h.n_ent) {
+            const int64_t e = (int64_t)h.ent_start + idx;
+            pinf[k] = ent_info[e];
+            if constexpr (MW == 1) {
+                pm[k] = ent_mask[e];
+            } else {
+                const uint2 w = reinterpret_cast<const uint2 *>(ent_mask)[e];
+                pm[k] = (uint64_t)w.x | (uint64_t)w.y << 32;
             }
         }
     }
-    for (int q = threadIdx.x; q < S; q += BT) stage[TE * S + q] = (T)0;
-    __syncthreads();
 
-    // A, compute: scaled gradients + coefficients of every tile element, one slot each
+    // ---- round 2: coordinate and coefficient gathers
+    double cx[NPT][3];
+#pragma unroll
+    for (int r = 0; r < NPT; r++) {
+        const int i = tid + r * BT;
+        cx[r][0] = cx[r][1] = cx[r][2] = 0.0;
+        if (i < NN) {
+            const double *p = coord + (int64_t)nodeId[r] * 3;
+            cx[r][0] = p[0];
+            cx[r][1] = p[1];
+            cx[r][2] = p[2];
+        }
+    }
+    double kc0[EPT], kc1[EPT];
 #pragma unroll
     for (int r = 0; r < EPT; r++) {
-        const int t = threadIdx.x + r * BT;
+        const int t = tid + r * BT;
+        kc0[r] = ELA ? 0.25 : 1.0;  // reference constants: lambda 0.25, mu 1 | k 1
+        kc1[r] = 1.0;
+        if (t < TE && coef) {
+            if constexpr (ELA) {
+                const double2 lm = reinterpret_cast<const double2 *>(coef)[elemId[r]];
+                kc0[r] = lm.x;
+                kc1[r] = lm.y;
+            } else {
+                kc0[r] = coef[elemId[r]];
+            }
+        }
+    }
+
+    // ---- static data into LDS
+#pragma unroll
+    for (int r = 0; r < NPT; r++) {
+        const int i = tid + r * BT;
+        if (i < R) {
+            lri[i] = rinf[r];
+            lrnode[i] = nodeId[r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < IPT; r++) {
+        const int i = tid + r * BT;
+        if (i < h.n_inc / 8) reinterpret_cast<uint4 *>(linc)[i] = incv[r];
+    }
+#pragma unroll
+    for (int r = 0; r < EPT; r++) {
+        const int t = tid + r * BT;
+        if (t < TE) lverts[t] = vert[r];
+    }
+#pragma unroll
+    for (int r = 0; r < NPT; r++) {
+        const int i = tid + r * BT;
+        if (i < NN) {
+            lc[i * 3 + 0] = cx[r][0];
+            lc[i * 3 + 1] = cx[r][1];
+            lc[i * 3 + 2] = cx[r][2];
+        }
+    }
+    __syncthreads();
+
+    // ---- A: scaled gradients + coefficients of every tile element, one slot each
+#pragma unroll
+    for (int r = 0; r < EPT; r++) {
+        const int t = tid + r * BT;
         if (t >= TE) continue;
-        const int vi[4] = {vert[r].x, vert[r].y, vert[r].z, vert[r].w};
+        const int vi[4] = {(int)(vert[r] & 0xFF), (int)((vert[r] >> 8) & 0xFF),
+                           (int)((vert[r] >> 16) & 0xFF), (int)(vert[r] >> 24)};
         double c[4][3], g[4][3];
 #pragma unroll
         for (int i = 0; i < 4; i++)
@@ -207,104 +255,158 @@ __global__ void __launch_bounds__(BT)
             *reinterpret_cast<T2 *>(sl + 2 * i) = T2{(T)g[i][0], (T)g[i][1]};
             sl[8 + i] = (T)g[i][2];
         }
-        if (OP == MF_OP_ELA) {
-            *reinterpret_cast<T2 *>(sl + 12) = T2{(T)kc[r][0], (T)kc[r][1]};
-            *reinterpret_cast<T2 *>(sl + 14) = T2{(T)kc[r][2], (T)0};
-            *reinterpret_cast<T2 *>(sl + 16) = T2{(T)0, (T)0};
+        if constexpr (ELA) {
+            // src/assembly.cc:386-409 with per-element (lambda, mu)
+            const double cd = kc0[r] + 2.0 * kc1[r], co = kc0[r] + kc1[r], cm = kc1[r];
+            *reinterpret_cast<T2 *>(sl + 12) = T2{(T)cd, (T)co};
+            *reinterpret_cast<T2 *>(sl + 14) = T2{(T)cm, (T)0};
         } else {
-            *reinterpret_cast<T2 *>(sl + 12) = T2{(T)kc[r][0], (T)0};
+            *reinterpret_cast<T2 *>(sl + 12) = T2{(T)kc0[r], (T)0};
         }
     }
     __syncthreads();
 
-    // B1: diagonal blocks (balanced chunks), kept in LDS, and their inverses. Padding
-    // entries point at the zero slot: they add +0.0, which leaves every sum unchanged.
-    for (int ch = wave; ch < h.n_dchunks; ch += NW) {
+    // ---- B1: diagonal blocks, one lane per row: the whole incidence list, ascending,
+    // four entries per LDS read with their stage reads issued together
+    if (wave < nDB) {
+        const int r = wave * 64 + lane;
+        if (r < R) {
+            const int4 ri = lri[r];
+            const int incOff = ri.z & 0xFFFF, val = (ri.z >> 16) & 0xFFFF;
+            const int64_t dpos = (int64_t)ri.x + ((ri.y >> 16) & 0xFF);
+            if constexpr (ELA) {
+                // ela_block(g, g): exactly symmetric (g_d*g_e == g_e*g_d in IEEE), so six
+                // sums hold the block bit for bit
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0, s4 = 0.0, s5 = 0.0, s8 = 0.0;
+                for (int i = 0; i < val; i += 4) {
+                    const uint2 e4 = *reinterpret_cast<const uint2 *>(linc + incOff + i);
+                    const int en[4] = {(int)(e4.x & 0xFFFF), (int)(e4.x >> 16),
+                                       (int)(e4.y & 0xFFFF), (int)(e4.y >> 16)};
+                    T g[4][3], cd[4], co[4], cm[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const T *sl = stage + (en[u] >> 2) * S;
+                        stage_grad<T, T2>(sl, en[u] & 3, g[u]);
+                        const T2 c01 = *reinterpret_cast<const T2 *>(sl + 12);
+                        cd[u] = c01.x;
+                        co[u] = c01.y;
+                        cm[u] = sl[14];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        if (i + u >= val) break;
+                        const T p00 = g[u][0] * g[u][0], p11 = g[u][1] * g[u][1],
+                                p22 = g[u][2] * g[u][2];
+                        s0 += (double)(p00 * cd[u] + p11 * cm[u] + p22 * cm[u]);
+                        s1 += (double)(g[u][0] * g[u][1] * co[u]);
+                        s2 += (double)(g[u][0] * g[u][2] * co[u]);
+                        s4 += (double)(p00 * cm[u] + p11 * cd[u] + p22 * cm[u]);
+                        s5 += (double)(g[u][1] * g[u][2] * co[u]);
+                        s8 += (double)(p00 * cm[u] + p11 * cm[u] + p22 * cd[u]);
+                    }
+                }
+                double blk[9] = {s0, s1, s2, s1, s4, s5, s2, s5, s8};
+                double *dv = values + dpos * 9;
+#pragma unroll
+                for (int q = 0; q < 9; q++) dv[q] = blk[q];
+                const int info = mf::ela_invert_block(blk, (ri.y >> 24) & 7, true);
+                double *dp = prec + (int64_t)lrnode[r] * 9;
+#pragma unroll
+                for (int q = 0; q < 9; q++) dp[q] = blk[q];
+                if (info != 0 && singular) atomicAdd(singular, 1);
+            } else {
+                double acc = 0.0;
+                for (int i = 0; i < val; i += 4) {
+                    const uint2 e4 = *reinterpret_cast<const uint2 *>(linc + incOff + i);
+                    const int en[4] = {(int)(e4.x & 0xFFFF), (int)(e4.x >> 16),
+                                       (int)(e4.y & 0xFFFF), (int)(e4.y >> 16)};
+                    T g[4][3], kk[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const T *sl = stage + (en[u] >> 2) * S;
+                        stage_grad<T, T2>(sl, en[u] & 3, g[u]);
+                        kk[u] = sl[12];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        if (i + u >= val) break;
+                        const T x = kk[u] * mf::lap_entry(g[u], g[u]);
+                        acc += (double)x;
+                    }
+                }
+                values[dpos] = acc;
+                prec[lrnode[r]] = 1.0 / acc;
+            }
+        }
+    }
+
+    // ---- B2: off-diagonal entries, one lane per CSR entry: the elements of the row's
+    // list that hold the column, two per iteration (independent LDS chains, summed in
+    // order)
+#pragma clang loop unroll(disable)
+    for (int k = 0; k < PB; k++) {
+        const int m = m0 + k * mstep;
+        if (m0 < 0 || m >= nEB) break;
+        const uint32_t inf = pinf[0];
+        M msk = pm[0];
+#pragma unroll
+        for (int s = 0; s + 1 < PB; s++) {
+            pinf[s] = pinf[s + 1];
+            pm[s] = pm[s + 1];
+        }
+        const int ra = inf & 0xFF, cl = (inf >> 8) & 0xFF;
+        if (inf == kNoEntry || cl == ra) continue;  // past the end, or the diagonal (B1)
+        const int4 ri = lri[ra];
+        const int incOff = ri.z & 0xFFFF;
+        const int oa = m * 64 + lane - ri.w;
         double acc[NA];
 #pragma unroll
         for (int q = 0; q < NA; q++) acc[q] = 0.0;
+        while (msk) {
+            const int b0 = lowest_bit(msk);
+            msk &= msk - 1;
+            const bool two = msk != 0;
+            const int b1 = two ? lowest_bit(msk) : b0;
+            if (two) msk &= msk - 1;
+            const int e0 = linc[incOff + b0], e1 = linc[incOff + b1];
+            const int t[2] = {e0 >> 2, e1 >> 2}, a[2] = {e0 & 3, e1 & 3};
+            T ga[2][3], gk[2][3], c0[2], c1[2], c2[2];
 #pragma unroll
-        for (int g = 0; g < kDiagPre; g++) {
-            if (g < dgroups) {
-                const uint32_t pk[2] = {dpre[g].x, dpre[g].y};
-                accumulate<OP, T, T2, S, GD>(stage, pk, acc);
+            for (int u = 0; u < 2; u++) {
+                const uint32_t vv = lverts[t[u]];
+                const int kk = (int)(vv & 0xFF) == cl ? 0
+                               : (int)((vv >> 8) & 0xFF) == cl ? 1
+                               : (int)((vv >> 16) & 0xFF) == cl ? 2 : 3;
+                const T *sl = stage + t[u] * S;
+                stage_grad<T, T2>(sl, a[u], ga[u]);
+                stage_grad<T, T2>(sl, kk, gk[u]);
+                if constexpr (ELA) {
+                    const T2 c01 = *reinterpret_cast<const T2 *>(sl + 12);
+                    c0[u] = c01.x;
+                    c1[u] = c01.y;
+                    c2[u] = sl[14];
+                } else {
+                    c0[u] = sl[12];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                if (u == 1 && !two) break;
+                if constexpr (ELA) {
+                    T bl[9];
+                    mf::ela_block(ga[u], gk[u], c0[u], c1[u], c2[u], bl);
+#pragma unroll
+                    for (int q = 0; q < 9; q++) acc[q] += (double)bl[q];
+                } else {
+                    const T x = c0[u] * mf::lap_entry(ga[u], gk[u]);
+                    acc[0] += (double)x;
+                }
             }
         }
-        for (int g = kDiagPre; g < dgroups; g++) {
-            const uint2 v = dcg[g * 64];
-            const uint32_t pk[2] = {v.x, v.y};
-            accumulate<OP, T, T2, S, GD>(stage, pk, acc);
-        }
-        const uint32_t w = dtask;
-        if (ch + NW < h.n_dchunks) dprefetch(ch + NW);
-        if (w == mf::kNoTask) continue;
-        const int ra = (int)w;
+        double *dst = values + ((int64_t)ri.x + oa) * NA;
 #pragma unroll
-        for (int q = 0; q < NA; q++) ldiag[ra * NA + q] = acc[q];
-        if constexpr (OP == MF_OP_ELA) {
-            double blk[9];
-#pragma unroll
-            for (int q = 0; q < 9; q++) blk[q] = acc[q];
-            const int info = mf::ela_invert_block(blk, lmask[ra], true);
-#pragma unroll
-            for (int q = 0; q < 9; q++) lprec[ra * 9 + q] = blk[q];
-            if (info != 0 && singular) atomicAdd(singular, 1);
-        } else {
-            lprec[ra] = 1.0 / acc[0];
-        }
+        for (int q = 0; q < NA; q++) dst[q] = acc[q];
     }
-    __syncthreads();
-
-    // B2: row chunks. The task word and the first PF contribution groups of the next
-    // chunk are loaded before this chunk's stores are issued.
-    int ch = wave;
-    uint32_t w = 0;
-    int ngroups = 0;
-    const uint32_t *cg = nullptr;
-    uint32_t pre[PF];
-    auto prefetch = [&](int c) {
-        const mf::ChunkHdr chh = chunks[h.rchunk_start + c];
-        w = tasks[(int64_t)(h.rchunk_start + c) * 64 + lane];
-        cg = reinterpret_cast<const uint32_t *>(contribs + chh.contrib_start) + lane;
-        ngroups = (chh.steps + GR - 1) / GR;
-#pragma unroll
-        for (int g = 0; g < PF; g++) pre[g] = g < ngroups ? cg[g * 64] : 0u;
-    };
-    if (ch < h.n_rchunks) prefetch(ch);
-    while (ch < h.n_rchunks) {
-        double acc[NA];
-#pragma unroll
-        for (int q = 0; q < NA; q++) acc[q] = 0.0;
-#pragma unroll
-        for (int g = 0; g < PF; g++) {
-            if (g < ngroups) {
-                const uint32_t pk[1] = {pre[g]};
-                accumulate<OP, T, T2, S, GR>(stage, pk, acc);
-            }
-        }
-        for (int g = PF; g < ngroups; g++) {
-            const uint32_t pk[1] = {cg[g * 64]};
-            accumulate<OP, T, T2, S, GR>(stage, pk, acc);
-        }
-        const uint32_t wc = w;
-        const int nch = ch + NW;
-        if (nch < h.n_rchunks) prefetch(nch);
-        if (wc != mf::kNoTask) {
-            const int ra = wc & 0xFF, oa = (wc >> 8) & 0xFF;
-            if (wc & mf::kDiagLane) {
-#pragma unroll
-                for (int q = 0; q < NA; q++) acc[q] = ldiag[ra * NA + q];
-            }
-            double *dst = values + ((int64_t)lrp[ra] + oa) * NA;
-#pragma unroll
-            for (int q = 0; q < NA; q++) dst[q] = acc[q];
-        }
-        ch = nch;
-    }
-
-    // C: the prec blocks of the tile's rows
-    for (int i = wave; i < R; i += NW)
-        if (lane < NA && lrl[i] > 0) prec[(int64_t)nodes[i] * NA + lane] = lprec[i * NA + lane];
 }
 
 // Nodes with an empty row: prec_init leaves zeros; lap inverts them to inf, ela masks
@@ -333,7 +435,7 @@ template <typename T>
 int upload(T **dst, const std::vector<T> &src)
 {
     *dst = nullptr;
-    const size_t bytes = std::max<size_t>(1, src.size()) * sizeof(T);
+    const size_t bytes = std::max<size_t>(16, src.size() * sizeof(T));
     if (hipMalloc(reinterpret_cast<void **>(dst), bytes) != hipSuccess) return MF_ERR_NOMEM;
     if (!src.empty() &&
         hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
@@ -341,28 +443,31 @@ int upload(T **dst, const std::vector<T> &src)
     return MF_OK;
 }
 
-template <typename T>
-int upload_raw(T **dst, const T *src, size_t n)
+template <typename D, typename S>
+int upload_as(D **dst, const std::vector<S> &src)
 {
-    return upload(dst, std::vector<T>(src, src + n));
+    static_assert(sizeof(D) % sizeof(S) == 0 || sizeof(S) % sizeof(D) == 0, "layout");
+    std::vector<D> tmp((src.size() * sizeof(S) + sizeof(D) - 1) / sizeof(D));
+    if (!src.empty()) std::memcpy(tmp.data(), src.data(), src.size() * sizeof(S));
+    return upload(dst, tmp);
 }
 
 void free_plan(mf_plan *p)
 {
     if (!p) return;
     void *ptrs[] = {p->tiles, p->tile_elems, p->tile_verts, p->tile_nodes, p->tile_rowinfo,
-                    p->chunks, p->tasks, p->contribs, p->cb, p->isolated};
+                    p->inc, p->ent_mask, p->ent_info, p->cb, p->isolated};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     delete p;
 }
 
-template <int OP, int PREC, int BT>
+template <int OP, int PREC, int BT, int MW>
 int launch_tiled(mf_plan *p, const double *coord, const double *coef, double *values,
                  double *prec, int *singular, hipStream_t s)
 {
     const int lds = PREC == MF_PREC_FP64 ? p->lds_fp64 : p->lds_mixed;
-    auto kern = k_tiled<OP, PREC, BT>;
+    auto kern = k_tiled<OP, PREC, BT, MW>;
     if (hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                             hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
         return MF_ERR_HIP;
@@ -370,7 +475,8 @@ int launch_tiled(mf_plan *p, const double *coord, const double *coef, double *va
     if (p->ntiles > 0) {
         hipLaunchKernelGGL(kern, dim3(per * kXcd), dim3(BT), lds, s, p->tiles, p->ntiles,
                            p->tile_elems, p->tile_verts, p->tile_nodes, p->tile_rowinfo,
-                           p->chunks, p->tasks, p->contribs, coord, coef, values, prec, singular);
+                           reinterpret_cast<const uint4 *>(p->inc), p->ent_mask, p->ent_info,
+                           coord, coef, values, prec, singular);
         MF_CHECK_LAUNCH();
     }
     if (p->n_isolated > 0) {
@@ -382,13 +488,17 @@ int launch_tiled(mf_plan *p, const double *coord, const double *coef, double *va
 }
 
 template <int OP, int PREC>
-int launch_bt(mf_plan *p, const double *coord, const double *coef, double *values,
-              double *prec, int *singular, hipStream_t s)
+int launch_variant(mf_plan *p, const double *coord, const double *coef, double *values,
+                   double *prec, int *singular, hipStream_t s)
 {
+    const bool wide = p->mask_words == 2;
     switch (p->block) {
-    case 128: return launch_tiled<OP, PREC, 128>(p, coord, coef, values, prec, singular, s);
-    case 256: return launch_tiled<OP, PREC, 256>(p, coord, coef, values, prec, singular, s);
-    case 512: return launch_tiled<OP, PREC, 512>(p, coord, coef, values, prec, singular, s);
+    case 256:
+        return wide ? launch_tiled<OP, PREC, 256, 2>(p, coord, coef, values, prec, singular, s)
+                    : launch_tiled<OP, PREC, 256, 1>(p, coord, coef, values, prec, singular, s);
+    case 512:
+        return wide ? launch_tiled<OP, PREC, 512, 2>(p, coord, coef, values, prec, singular, s)
+                    : launch_tiled<OP, PREC, 512, 1>(p, coord, coef, values, prec, singular, s);
     default: return MF_ERR_ARG;
     }
 }
@@ -410,7 +520,7 @@ int resolve_params(int op, const mf_plan_params *params, Resolved *r)
     }
     r->slot64 = S * 8;
     r->slot32 = S * 4;
-    if (r->block != 128 && r->block != 256 && r->block != 512) return MF_ERR_ARG;
+    if (r->block != 256 && r->block != 512) return MF_ERR_ARG;
     if (r->budget < 4096 || r->budget > mf::kMaxLdsBytes || r->max_rows > mf::kMaxTileRows)
         return MF_ERR_ARG;
     return MF_OK;
@@ -421,16 +531,14 @@ int build_checked(mf::PlanHost &h, int op, int nE, int N, int nnz, const int *e2
                   const Resolved &r, int *lds64, int *lds32)
 {
     int st = mf::build_plan(h, op, nE, N, nnz, e2n, row, col, cb, coord, r.budget, r.max_rows,
-                            r.slot64);
+                            r.slot64, r.block);
     if (st != MF_OK) return st;
-    st = mf::validate_plan(h, row, col, r.slot64);
+    st = mf::validate_plan(h, row, col, e2n, r.slot64);
     if (st != MF_OK) return st;
-    const int bb = op == MF_OP_ELA ? 72 : 8;
     int m64 = 0, m32 = 0;
     for (const auto &t : h.tiles) {
-        if (t.n_elem > 8 * r.block) return MF_ERR_ARG;  // the kernel keeps <= 8 per thread
-        m64 = std::max(m64, mf::tile_lds_layout(t.n_node, t.n_elem, t.n_rows, r.slot64, bb).total);
-        m32 = std::max(m32, mf::tile_lds_layout(t.n_node, t.n_elem, t.n_rows, r.slot32, bb).total);
+        m64 = std::max(m64, mf::tile_lds_layout(t.n_node, t.n_elem, t.n_rows, t.n_inc, r.slot64).total);
+        m32 = std::max(m32, mf::tile_lds_layout(t.n_node, t.n_elem, t.n_rows, t.n_inc, r.slot32).total);
     }
     if (m64 + 16 > mf::kMaxLdsBytes) return MF_ERR_ARG;  // one row needs more than a CU has
     *lds64 = m64 + 16;
@@ -438,17 +546,18 @@ int build_checked(mf::PlanHost &h, int op, int nE, int N, int nnz, const int *e2
     return MF_OK;
 }
 
+// stats: tiles, staged tile elements, CSR entries, incidence entries, plan bytes, LDS
+// bytes per workgroup (FP64), block threads, largest tile (rows)
 void fill_stats(const mf::PlanHost &h, int lds64, int block, int64_t out[8])
 {
     out[0] = (int64_t)h.tiles.size();
     out[1] = (int64_t)h.tile_elems.size();
-    out[2] = h.total_tasks;
-    out[3] = (int64_t)h.contribs.size();
+    out[2] = (int64_t)h.ent_info.size();
+    out[3] = (int64_t)h.inc.size();
     out[4] = (int64_t)h.tiles.size() * sizeof(mf::TileHdr) + (int64_t)h.tile_elems.size() * 4 +
              (int64_t)h.tile_verts.size() + (int64_t)h.tile_nodes.size() * 4 +
-             (int64_t)h.tile_rowinfo.size() * sizeof(mf::RowInfo) +
-             (int64_t)h.chunks.size() * sizeof(mf::ChunkHdr) + (int64_t)h.tasks.size() * 4 +
-             (int64_t)h.contribs.size() * 2;
+             (int64_t)h.tile_rowinfo.size() * sizeof(mf::RowInfo) + (int64_t)h.inc.size() * 2 +
+             (int64_t)h.ent_mask.size() * 4 + (int64_t)h.ent_info.size() * 2;
     out[5] = lds64;
     out[6] = block;
     out[7] = h.max_rows;
@@ -488,22 +597,21 @@ extern "C" int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nb
     p->dim = operatorID == MF_OP_ELA ? 9 : 1;
     p->ntiles = (int)h.tiles.size();
     p->block = r.block;
+    p->mask_words = h.mask_words;
     p->lds_fp64 = lds64;
     p->lds_mixed = lds32;
     p->n_isolated = (int)h.isolated.size();
-    std::vector<uchar4> verts(h.tile_verts.size() / 4);
-    std::memcpy(verts.data(), h.tile_verts.data(), h.tile_verts.size());
     int rc = upload(&p->tiles, h.tiles);
     if (rc == MF_OK) rc = upload(&p->tile_elems, h.tile_elems);
-    if (rc == MF_OK) rc = upload(&p->tile_verts, verts);
+    if (rc == MF_OK) rc = upload_as(&p->tile_verts, h.tile_verts);
     if (rc == MF_OK) rc = upload(&p->tile_nodes, h.tile_nodes);
-    if (rc == MF_OK) rc = upload(&p->tile_rowinfo, h.tile_rowinfo);
-    if (rc == MF_OK) rc = upload(&p->chunks, h.chunks);
-    if (rc == MF_OK) rc = upload(&p->tasks, h.tasks);
-    if (rc == MF_OK) rc = upload(&p->contribs, h.contribs);
+    if (rc == MF_OK) rc = upload_as(&p->tile_rowinfo, h.tile_rowinfo);
+    if (rc == MF_OK) rc = upload(&p->inc, h.inc);
+    if (rc == MF_OK) rc = upload(&p->ent_mask, h.ent_mask);
+    if (rc == MF_OK) rc = upload(&p->ent_info, h.ent_info);
     if (rc == MF_OK) rc = upload(&p->isolated, h.isolated);
     if (rc == MF_OK && checkBounds && operatorID == MF_OP_ELA)
-        rc = upload_raw(&p->cb, checkBounds, (size_t)nbNodes * 3);
+        rc = upload(&p->cb, std::vector<int32_t>(checkBounds, checkBounds + (size_t)nbNodes * 3));
     if (rc != MF_OK) {
         free_plan(p);
         return rc;
@@ -554,9 +662,9 @@ extern "C" int mf_plan_assemble(mf_plan *plan, const double *coord, const double
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (plan->op == MF_OP_LAP)
         return precision == MF_PREC_FP64
-                   ? launch_bt<MF_OP_LAP, MF_PREC_FP64>(plan, coord, coef, values, prec, singular, s)
-                   : launch_bt<MF_OP_LAP, MF_PREC_MIXED>(plan, coord, coef, values, prec, singular, s);
+                   ? launch_variant<MF_OP_LAP, MF_PREC_FP64>(plan, coord, coef, values, prec, singular, s)
+                   : launch_variant<MF_OP_LAP, MF_PREC_MIXED>(plan, coord, coef, values, prec, singular, s);
     return precision == MF_PREC_FP64
-               ? launch_bt<MF_OP_ELA, MF_PREC_FP64>(plan, coord, coef, values, prec, singular, s)
-               : launch_bt<MF_OP_ELA, MF_PREC_MIXED>(plan, coord, coef, values, prec, singular, s);
+               ? launch_variant<MF_OP_ELA, MF_PREC_FP64>(plan, coord, coef, values, prec, singular, s)
+               : launch_variant<MF_OP_ELA, MF_PREC_MIXED>(plan, coord, coef, values, prec, singular, s);
 }

@@ -1,4 +1,4 @@
-// Host construction and validation of the tiled gather schedule (see plan.h).
+// Host construction and validation of the tiled gather schedule (layout in plan.h).
 #include <algorithm>
 #include <climits>
 #include <cstdint>
@@ -14,7 +14,8 @@ namespace mf {
 namespace {
 
 struct Partitioner {
-    int N, nE, max_rows, budget, eb, bb;
+    int N, nE, max_rows, budget, slot, block;
+    bool ela;
     const int *e2n, *row;
     const std::vector<int> &n2eI, &n2eV;
     const double *coord;
@@ -23,25 +24,28 @@ struct Partitioner {
     std::vector<int> elemStamp, nodeStamp;
     int stampId = 0;
 
-    Partitioner(int N_, int nE_, int mr, int bud, int eb_, int bb_, const int *e2n_,
-                const int *row_, const std::vector<int> &I, const std::vector<int> &V,
-                const double *c)
-        : N(N_), nE(nE_), max_rows(mr), budget(bud), eb(eb_), bb(bb_), e2n(e2n_), row(row_),
-          n2eI(I), n2eV(V), coord(c), order(N_), elemStamp(nE_, -1), nodeStamp(N_, -1)
+    Partitioner(int N_, int nE_, int mr, int bud, int slot_, int block_, bool ela_,
+                const int *e2n_, const int *row_, const std::vector<int> &I,
+                const std::vector<int> &V, const double *c, std::vector<int> rows)
+        : N(N_), nE(nE_), max_rows(mr), budget(bud), slot(slot_), block(block_), ela(ela_),
+          e2n(e2n_), row(row_), n2eI(I), n2eV(V), coord(c), order(std::move(rows)),
+          elemStamp(nE_, -1), nodeStamp(N_, -1)
     {
-        std::iota(order.begin(), order.end(), 0);
     }
 
-    // Exact LDS bytes of a tile made of rows order[lo, hi), or INT32_MAX when it exceeds
-    // the element, node or row limits.
-    int64_t lds_of(int lo, int hi)
+    // Whether the tile made of rows order[lo, hi) fits the LDS budget and every
+    // per-tile limit of the kernel.
+    bool fits(int lo, int hi)
     {
+        const int R = hi - lo;
+        if (R > max_rows || R > kMaxTileRows) return false;
         stampId++;
         for (int i = lo; i < hi; i++) nodeStamp[order[i]] = stampId;
-        int64_t TE = 0, NN = hi - lo;
+        int64_t TE = 0, NN = R, inc = 0, ent = 0;
         for (int i = lo; i < hi; i++) {
             const int r = order[i];
-            if (row[r + 1] - row[r] > kMaxRowLen) return INT32_MAX;
+            ent += row[r + 1] - row[r];
+            inc += (n2eI[r + 1] - n2eI[r] + kIncAlign - 1) / kIncAlign * kIncAlign;
             for (int j = n2eI[r]; j < n2eI[r + 1]; j++) {
                 const int e = n2eV[j];
                 if (elemStamp[e] == stampId) continue;
@@ -53,11 +57,13 @@ struct Partitioner {
                 }
             }
         }
-        if (NN > kMaxTileNodes || TE > kMaxTileElems || hi - lo > kMaxTileRows) return INT32_MAX;
-        return tile_lds_layout((int)NN, (int)TE, hi - lo, eb, bb).total;
+        if (NN > kMaxTileNodes || TE > kMaxTileElems || TE > (int64_t)elems_per_thread(block) * block)
+            return false;
+        const int64_t inc8 = (inc + 7) / 8 * 8;
+        if (inc8 > kMaxTileInc) return false;
+        if (diag_blocks(ela, R) + (ent + kLanes - 1) / kLanes > kMaxTileBlocks) return false;
+        return tile_lds_layout((int)NN, (int)TE, R, (int)inc8, slot).total <= budget;
     }
-
-    bool fits(int lo, int hi) { return hi - lo <= max_rows && lds_of(lo, hi) <= budget; }
 
     void rcb(int lo, int hi)
     {
@@ -90,70 +96,60 @@ struct Partitioner {
     // chunks that fit.
     void bfs_chunks()
     {
-        std::vector<char> seen(N, 0);
-        int head = 0, tail = 0;
-        for (int s = 0; s < N; s++) {
+        std::vector<char> isRow(N, 0), seen(N, 0);
+        for (int r : order) isRow[r] = 1;
+        std::vector<int> q;
+        q.reserve(order.size());
+        for (int s : std::vector<int>(order)) {
             if (seen[s]) continue;
             seen[s] = 1;
-            order[tail++] = s;
-            while (head < tail) {
-                const int r = order[head++];
+            size_t head = q.size();
+            q.push_back(s);
+            while (head < q.size()) {
+                const int r = q[head++];
                 for (int j = n2eI[r]; j < n2eI[r + 1]; j++) {
                     const int e = n2eV[j];
                     for (int k = 0; k < 4; k++) {
                         const int v = e2n[(int64_t)e * 4 + k] - 1;
-                        if (!seen[v]) { seen[v] = 1; order[tail++] = v; }
+                        if (!seen[v] && isRow[v]) { seen[v] = 1; q.push_back(v); }
                     }
                 }
             }
         }
+        order = q;
+        const int n = (int)order.size();
         int lo = 0;
-        while (lo < N) {
+        while (lo < n) {
             int hi = lo + 1;
-            while (hi < N && fits(lo, hi + 1)) hi++;
+            while (hi < n && fits(lo, hi + 1)) hi++;
             leaves.emplace_back(lo, hi);
             lo = hi;
         }
     }
 };
 
-// Appends one chunk's contributions in the grouped, lane-interleaved layout.
-void emit_chunk(PlanHost &p, int group, int te, const std::vector<const uint16_t *> &lists,
-                const std::vector<int> &counts)
-{
-    ChunkHdr c{};
-    c.contrib_start = (int)p.contribs.size();
-    c.steps = 0;
-    for (int n : counts) c.steps = std::max(c.steps, n);
-    const int groups = (c.steps + group - 1) / group;
-    for (int g = 0; g < groups; g++)
-        for (int l = 0; l < kTaskLanes; l++)
-            for (int j = 0; j < group; j++) {
-                const int s = g * group + j;
-                uint16_t en = (uint16_t)(te << 4);  // padding: the zero element slot
-                if (l < (int)counts.size() && s < counts[l]) en = lists[l][s];
-                p.contribs.push_back(en);
-            }
-    p.chunks.push_back(c);
-}
-
 }  // namespace
 
 int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, const int *row,
                const int *col, const int *checkBounds, const double *coord, int lds_budget,
-               int max_rows, int slot_bytes)
+               int max_rows, int slot_bytes, int block)
 {
     p = PlanHost{};
     p.op = op;
     p.nE = nE;
     p.N = N;
     p.nnz = nnz;
+    p.block = block;
+    const bool ela = op == MF_OP_ELA;
     if (nE < 0 || N < 0 || nnz < 0 || max_rows < 1 || max_rows > kMaxTileRows) return MF_ERR_ARG;
+    if (block < kLanes || block % kLanes || kMaxTileBlocks % (block / kLanes)) return MF_ERR_ARG;
     if (row[0] != 0 || row[N] != nnz) return MF_ERR_ARG;
+    std::vector<int> rowsAll;
     for (int i = 0; i < N; i++) {
         const int len = row[i + 1] - row[i];
-        if (len < 0 || len > kMaxRowLen) return MF_ERR_ARG;
+        if (len < 0 || len > 0xFFFF) return MF_ERR_ARG;
         if (len == 0) p.isolated.push_back(i);
+        else rowsAll.push_back(i);
     }
     std::vector<int> n2eI((size_t)N + 1, 0), n2eV((size_t)nE * 4 + 1);
     for (int64_t q = 0; q < (int64_t)nE * 4; q++) {
@@ -161,26 +157,30 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
         if (v < 0 || v >= N) return MF_ERR_ARG;
         n2eI[v + 1]++;
     }
-    for (int i = 0; i < N; i++) n2eI[i + 1] += n2eI[i];
+    int maxVal = 0;
+    for (int i = 0; i < N; i++) {
+        maxVal = std::max(maxVal, n2eI[i + 1]);
+        n2eI[i + 1] += n2eI[i];
+    }
+    if (maxVal > kMaxValence) return MF_ERR_ARG;
+    p.mask_words = maxVal > 32 ? 2 : 1;
     {
         std::vector<int> cur(n2eI.begin(), n2eI.end() - 1);
         for (int e = 0; e < nE; e++)
             for (int k = 0; k < 4; k++) n2eV[cur[e2n[(int64_t)e * 4 + k] - 1]++] = e;
     }
-    const int bb = op == MF_OP_ELA ? 72 : 8;
-    Partitioner part(N, nE, max_rows, lds_budget, slot_bytes, bb, e2n, row, n2eI, n2eV, coord);
-    if (coord) part.rcb(0, N);
+    // a node with elements has a non-empty row holding its diagonal
+    for (int i = 0; i < N; i++)
+        if ((n2eI[i + 1] > n2eI[i]) != (row[i + 1] > row[i])) return MF_ERR_ARG;
+
+    Partitioner part(N, nE, max_rows, lds_budget, slot_bytes, block, ela, e2n, row, n2eI, n2eV,
+                     coord, rowsAll);
+    if (coord) part.rcb(0, (int)rowsAll.size());
     else part.bfs_chunks();
 
     std::vector<int> nodeLocal((size_t)N, -1), elemLocal((size_t)nE, -1);
     std::vector<int> rows, tel;
-    // per-row contribution lists of the tile: diag list + one list per CSR entry
-    std::vector<uint16_t> buf;
-    std::vector<int> diagStart, diagCnt, entStart, entCnt;
-    std::vector<int> dorder;
-    std::vector<const uint16_t *> lists;
-    std::vector<int> counts;
-
+    const int MW = p.mask_words;
     for (const auto &lf : part.leaves) {
         rows.assign(part.order.begin() + lf.first, part.order.begin() + lf.second);
         std::sort(rows.begin(), rows.end());
@@ -191,14 +191,8 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
         h.n_rows = R;
         h.rinfo_start = (int)p.tile_rowinfo.size();
         for (int i = 0; i < R; i++) {
-            const int a = rows[i];
-            nodeLocal[a] = i;
-            p.tile_nodes.push_back(a);
-            int mask = 0;
-            if (op == MF_OP_ELA && checkBounds)
-                for (int k = 0; k < 3; k++)
-                    if (checkBounds[(int64_t)k * N + a] != 0) mask |= 1 << k;
-            p.tile_rowinfo.push_back(RowInfo{row[a], (row[a + 1] - row[a]) | mask << 16});
+            nodeLocal[rows[i]] = i;
+            p.tile_nodes.push_back(rows[i]);
         }
         tel.clear();
         for (int r : rows)
@@ -228,133 +222,105 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
                 p.tile_verts.push_back((uint8_t)nodeLocal[e2n[(int64_t)tel[t] * 4 + k] - 1]);
         }
 
-        // contribution lists, ascending tile element order (= ascending element id)
-        buf.clear();
-        diagStart.assign(R, 0);
-        diagCnt.assign(R, 0);
-        entStart.clear();
-        entCnt.clear();
-        for (int ra = 0; ra < R; ra++) {
-            const int a = rows[ra];
-            diagStart[ra] = (int)buf.size();
-            for (int j = n2eI[a]; j < n2eI[a + 1]; j++) {
-                const int e = n2eV[j];
-                const int *v = e2n + (int64_t)e * 4;
-                int va = 0;
-                for (int k = 0; k < 4; k++)
-                    if (v[k] - 1 == a) va = k;
-                buf.push_back((uint16_t)(elemLocal[e] << 4 | va << 2 | va));
-            }
-            diagCnt[ra] = (int)buf.size() - diagStart[ra];
-            for (int pa = row[a]; pa < row[a + 1]; pa++) {
-                const int c = col[pa] - 1;
-                entStart.push_back((int)buf.size());
-                if (c != a) {
-                    for (int j = n2eI[a]; j < n2eI[a + 1]; j++) {
-                        const int e = n2eV[j];
-                        const int *v = e2n + (int64_t)e * 4;
-                        int va = -1, vc = -1;
-                        for (int k = 0; k < 4; k++) {
-                            if (v[k] - 1 == a) va = k;
-                            if (v[k] - 1 == c) vc = k;
-                        }
-                        if (vc < 0) continue;
-                        buf.push_back((uint16_t)(elemLocal[e] << 4 | va << 2 | vc));
-                    }
-                }
-                entCnt.push_back((int)buf.size() - entStart.back());
-            }
-        }
-
-        // B1 diagonal chunks: rows with a diagonal, by contribution count (balanced waves)
-        dorder.clear();
-        for (int ra = 0; ra < R; ra++)
-            if (row[rows[ra] + 1] > row[rows[ra]]) dorder.push_back(ra);
-        std::stable_sort(dorder.begin(), dorder.end(),
-                         [&](int x, int y) { return diagCnt[x] > diagCnt[y]; });
-        h.dchunk_start = (int)p.chunks.size();
-        for (size_t i = 0; i < dorder.size(); i += kTaskLanes) {
-            lists.clear();
-            counts.clear();
-            for (size_t l = 0; l < (size_t)kTaskLanes; l++) {
-                if (i + l < dorder.size()) {
-                    const int ra = dorder[i + l];
-                    lists.push_back(buf.data() + diagStart[ra]);
-                    counts.push_back(diagCnt[ra]);
-                    p.tasks.push_back((uint32_t)ra);
-                } else {
-                    p.tasks.push_back(kNoTask);
-                }
-            }
-            emit_chunk(p, kDiagGroup, TE, lists, counts);
-        }
-        h.n_dchunks = (int)p.chunks.size() - h.dchunk_start;
-
-        // B2 row chunks: whole rows in tile-row order, one lane per CSR entry; a row longer
-        // than the lanes left starts a new chunk (or spans chunks if longer than 64)
-        h.rchunk_start = (int)p.chunks.size();
-        lists.clear();
-        counts.clear();
-        std::vector<uint32_t> words;
-        auto flush = [&]() {
-            if (words.empty()) return;
-            for (int l = 0; l < kTaskLanes; l++)
-                p.tasks.push_back(l < (int)words.size() ? words[l] : kNoTask);
-            emit_chunk(p, kRowGroup, TE, lists, counts);
-            lists.clear();
-            counts.clear();
-            words.clear();
-        };
-        int ent = 0;
+        // incidence lists (ascending element = ascending tile element) and entries
+        h.inc_start = (int)p.inc.size();
+        h.ent_start = (int)p.ent_info.size();
+        int incOff = 0, entOff = 0;
         for (int ra = 0; ra < R; ra++) {
             const int a = rows[ra], len = row[a + 1] - row[a];
-            if (len <= kTaskLanes && (int)words.size() + len > kTaskLanes) flush();
-            for (int oa = 0; oa < len; oa++, ent++) {
-                if ((int)words.size() == kTaskLanes) flush();
-                const bool diag = col[row[a] + oa] - 1 == a;
-                words.push_back((uint32_t)ra | (uint32_t)oa << 8 | (diag ? kDiagLane : 0u));
-                lists.push_back(buf.data() + entStart[ent]);
-                counts.push_back(entCnt[ent]);
+            const int val = n2eI[a + 1] - n2eI[a];
+            int doff = -1;
+            for (int o = 0; o < len; o++)
+                if (col[row[a] + o] - 1 == a) { doff = o; break; }
+            if (doff < 0 || doff > 0xFF) return MF_ERR_ARG;
+            int mask = 0;
+            if (ela && checkBounds)
+                for (int k = 0; k < 3; k++)
+                    if (checkBounds[(int64_t)k * N + a] != 0) mask |= 1 << k;
+            p.tile_rowinfo.push_back(
+                RowInfo{row[a], len | doff << 16 | mask << 24, incOff | val << 16, entOff});
+            for (int j = n2eI[a]; j < n2eI[a + 1]; j++) {
+                const int e = n2eV[j];
+                int va = 0;
+                for (int k = 0; k < 4; k++)
+                    if (e2n[(int64_t)e * 4 + k] - 1 == a) va = k;
+                p.inc.push_back((uint16_t)(elemLocal[e] << 2 | va));
             }
+            for (int o = 0; o < len; o++) {
+                const int c = col[row[a] + o] - 1;
+                if (c < 0 || c >= N || nodeLocal[c] < 0) return MF_ERR_ARG;
+                uint64_t m = 0;
+                if (c != a)
+                    for (int j = n2eI[a]; j < n2eI[a + 1]; j++) {
+                        const int *v = e2n + (int64_t)n2eV[j] * 4;
+                        if (v[0] - 1 == c || v[1] - 1 == c || v[2] - 1 == c || v[3] - 1 == c)
+                            m |= 1ull << (j - n2eI[a]);
+                    }
+                p.ent_info.push_back((uint16_t)(ra | nodeLocal[c] << 8));
+                for (int w = 0; w < MW; w++) p.ent_mask.push_back((uint32_t)(m >> (32 * w)));
+            }
+            while (p.inc.size() % kIncAlign) p.inc.push_back(0);
+            incOff = (int)p.inc.size() - h.inc_start;
+            entOff += len;
         }
-        flush();
-        h.n_rchunks = (int)p.chunks.size() - h.rchunk_start;
-
-        p.total_tasks += (int64_t)dorder.size() + ent;
+        while (p.inc.size() % 8) p.inc.push_back(0);
+        h.n_inc = (int)p.inc.size() - h.inc_start;
+        h.n_ent = entOff;
         p.tiles.push_back(h);
         p.max_node = std::max(p.max_node, h.n_node);
         p.max_elem = std::max(p.max_elem, TE);
         p.max_rows = std::max(p.max_rows, R);
         for (int i = h.node_start; i < (int)p.tile_nodes.size(); i++) nodeLocal[p.tile_nodes[i]] = -1;
         for (int e : tel) elemLocal[e] = -1;
-        if (p.contribs.size() > (size_t)INT32_MAX - 1000000) return MF_ERR_NOMEM;
+        if (p.inc.size() > (size_t)INT32_MAX - 100000 || p.ent_info.size() > (size_t)INT32_MAX / 2)
+            return MF_ERR_NOMEM;
     }
     return MF_OK;
 }
 
-int validate_plan(const PlanHost &p, const int *row, const int *col, int slot_bytes)
+int validate_plan(const PlanHost &p, const int *row, const int *col, const int *e2n,
+                  int slot_bytes)
 {
-    std::vector<uint8_t> hit((size_t)p.nnz, 0);
-    std::vector<uint8_t> rowSeen((size_t)p.N, 0);
-    const int bb = p.op == MF_OP_ELA ? 72 : 8;
-    if (p.tasks.size() != p.chunks.size() * kTaskLanes) return MF_ERR_STATE;
+    const bool ela = p.op == MF_OP_ELA;
+    const int MW = p.mask_words;
+    if (MW < 1 || MW > 2) return MF_ERR_STATE;
     if (p.tile_verts.size() != p.tile_elems.size() * 4) return MF_ERR_STATE;
+    if (p.ent_mask.size() != p.ent_info.size() * MW) return MF_ERR_STATE;
+    std::vector<uint8_t> hit((size_t)p.nnz, 0), rowSeen((size_t)p.N, 0);
+    for (int i : p.isolated) {
+        if (i < 0 || i >= p.N || row[i + 1] != row[i] || rowSeen[i]) return MF_ERR_STATE;
+        rowSeen[i] = 1;
+    }
+    const int NW = p.block / kLanes;
+    std::vector<int> nodeStamp((size_t)p.N, -1), cnt(kMaxTileNodes);
+    std::vector<uint64_t> holds(kMaxTileNodes, 0);
+    int tileNo = 0;
     for (const TileHdr &h : p.tiles) {
-        if (h.n_rows < 1 || h.n_rows > kMaxTileRows || h.n_node < h.n_rows ||
-            h.n_node > kMaxTileNodes || h.n_elem < 0 || h.n_elem > kMaxTileElems)
+        const int R = h.n_rows;
+        if (R < 1 || R > kMaxTileRows || h.n_node < R || h.n_node > kMaxTileNodes ||
+            h.n_elem < 0 || h.n_elem > kMaxTileElems ||
+            h.n_elem > elems_per_thread(p.block) * p.block)
             return MF_ERR_STATE;
         if (h.node_start < 0 || (size_t)h.node_start + h.n_node > p.tile_nodes.size()) return MF_ERR_STATE;
         if (h.elem_start < 0 || (size_t)h.elem_start + h.n_elem > p.tile_elems.size()) return MF_ERR_STATE;
-        if (h.dchunk_start < 0 || h.n_dchunks < 0 || h.rchunk_start < 0 || h.n_rchunks < 0 ||
-            (size_t)h.dchunk_start + h.n_dchunks > p.chunks.size() ||
-            (size_t)h.rchunk_start + h.n_rchunks > p.chunks.size())
+        if (h.rinfo_start < 0 || (size_t)h.rinfo_start + R > p.tile_rowinfo.size()) return MF_ERR_STATE;
+        if (h.inc_start < 0 || h.inc_start % 8 || h.n_inc < 0 || h.n_inc % 8 ||
+            h.n_inc > kMaxTileInc || (size_t)h.inc_start + h.n_inc > p.inc.size())
             return MF_ERR_STATE;
-        if (tile_lds_layout(h.n_node, h.n_elem, h.n_rows, slot_bytes, bb).total + 16 > kMaxLdsBytes)
+        if (h.ent_start < 0 || h.n_ent < 0 || (size_t)h.ent_start + h.n_ent > p.ent_info.size())
             return MF_ERR_STATE;
+        const int blocks = diag_blocks(ela, R) + (h.n_ent + kLanes - 1) / kLanes;
+        if (blocks > kMaxTileBlocks || (blocks + NW - 1) / NW > kMaxTileBlocks / NW) return MF_ERR_STATE;
+        if (tile_lds_layout(h.n_node, h.n_elem, R, h.n_inc, slot_bytes).total + 16 > kMaxLdsBytes)
+            return MF_ERR_STATE;
+        const int32_t *nodes = p.tile_nodes.data() + h.node_start;
+        const uint8_t *verts = p.tile_verts.data() + (size_t)h.elem_start * 4;
+        tileNo++;
         for (int i = 0; i < h.n_node; i++) {
-            const int n = p.tile_nodes[h.node_start + i];
-            if (n < 0 || n >= p.N) return MF_ERR_STATE;
-            if (i < h.n_rows) {
+            const int n = nodes[i];
+            if (n < 0 || n >= p.N || nodeStamp[n] == tileNo) return MF_ERR_STATE;
+            nodeStamp[n] = tileNo;
+            if (i < R) {
                 if (rowSeen[n]) return MF_ERR_STATE;
                 rowSeen[n] = 1;
             }
@@ -364,66 +330,55 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, int slot_by
             if (e < 0 || e >= p.nE) return MF_ERR_STATE;
             if (t > 0 && e <= p.tile_elems[h.elem_start + t - 1]) return MF_ERR_STATE;
             for (int k = 0; k < 4; k++)
-                if (p.tile_verts[(size_t)(h.elem_start + t) * 4 + k] >= h.n_node) return MF_ERR_STATE;
-        }
-        if (h.rinfo_start < 0 || (size_t)h.rinfo_start + h.n_rows > p.tile_rowinfo.size())
-            return MF_ERR_STATE;
-        for (int i = 0; i < h.n_rows; i++) {
-            const int n = p.tile_nodes[h.node_start + i];
-            const RowInfo ri = p.tile_rowinfo[h.rinfo_start + i];
-            if (ri.rowptr != row[n] || (ri.len_mask & 0xFFFF) != row[n + 1] - row[n] ||
-                (ri.len_mask >> 16) > 7)
-                return MF_ERR_STATE;
-        }
-        std::vector<int> diagSeen(h.n_rows, 0);
-        for (int pass = 0; pass < 2; pass++) {
-            const int c0 = pass == 0 ? h.dchunk_start : h.rchunk_start;
-            const int nc = pass == 0 ? h.n_dchunks : h.n_rchunks;
-            const int G = pass == 0 ? kDiagGroup : kRowGroup;
-            for (int ch = c0; ch < c0 + nc; ch++) {
-                const ChunkHdr &c = p.chunks[ch];
-                const int groups = (c.steps + G - 1) / G;
-                if (c.steps < 0 || c.contrib_start < 0 || c.contrib_start % G ||
-                    (int64_t)c.contrib_start + (int64_t)groups * G * kTaskLanes >
-                        (int64_t)p.contribs.size())
+                if (verts[t * 4 + k] >= h.n_node || nodes[verts[t * 4 + k]] != e2n[(int64_t)e * 4 + k] - 1)
                     return MF_ERR_STATE;
-                for (int l = 0; l < kTaskLanes; l++) {
-                    int last = -1;
-                    for (int s = 0; s < groups * G; s++) {
-                        const uint16_t en = p.contribs[(size_t)c.contrib_start +
-                                                       (size_t)(s / G) * G * kTaskLanes +
-                                                       l * G + s % G];
-                        const int t = en >> 4;
-                        if (t == h.n_elem) {  // padding, only after the lane's real entries
-                            if (en != (uint16_t)(h.n_elem << 4)) return MF_ERR_STATE;
-                            last = INT_MAX;
-                            continue;
-                        }
-                        if (t > h.n_elem || t <= last) return MF_ERR_STATE;
-                        last = t;
-                    }
-                    const uint32_t w = p.tasks[(size_t)ch * kTaskLanes + l];
-                    if (w == kNoTask) continue;
-                    if (pass == 0) {
-                        if ((int)w >= h.n_rows || diagSeen[w]++) return MF_ERR_STATE;
-                        const int a = p.tile_nodes[h.node_start + w];
-                        if (row[a + 1] == row[a]) return MF_ERR_STATE;
-                        continue;
-                    }
-                    const int ra = w & 0xFF, oa = (w >> 8) & 0xFF;
-                    if ((w & ~(kDiagLane | 0xFFFFu)) || ra >= h.n_rows) return MF_ERR_STATE;
-                    const int a = p.tile_nodes[h.node_start + ra];
-                    if (oa >= row[a + 1] - row[a]) return MF_ERR_STATE;
-                    const int pos = row[a] + oa;
-                    if (hit[pos]++) return MF_ERR_STATE;
-                    if (((w & kDiagLane) != 0) != (col[pos] == a + 1)) return MF_ERR_STATE;
-                }
+        }
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (int q = 0; q < h.n_elem * 4; q++) cnt[verts[q]]++;
+        int incNext = 0, entNext = 0;
+        for (int ra = 0; ra < R; ra++) {
+            const int a = nodes[ra];
+            const RowInfo ri = p.tile_rowinfo[h.rinfo_start + ra];
+            const int len = ri.len_diag & 0xFFFF, doff = (ri.len_diag >> 16) & 0xFF;
+            const int mask = (ri.len_diag >> 24) & 0xFF;
+            const int incOff = ri.inc & 0xFFFF, val = (ri.inc >> 16) & 0xFFFF;
+            if (ri.rowptr != row[a] || len != row[a + 1] - row[a] || len < 1 || doff >= len ||
+                col[row[a] + doff] != a + 1 || mask > 7 || (!ela && mask))
+                return MF_ERR_STATE;
+            if (incOff != incNext || incOff % kIncAlign || val < 1 || val > 32 * MW || incOff + val > h.n_inc ||
+                ri.ent != entNext || ri.ent + len > h.n_ent)
+                return MF_ERR_STATE;
+            incNext += (val + kIncAlign - 1) / kIncAlign * kIncAlign;
+            entNext += len;
+            // the incidence list: every element of the tile holding a, ascending
+            int last = -1;
+            for (int i = 0; i < val; i++) {
+                const uint16_t en = p.inc[(size_t)h.inc_start + incOff + i];
+                const int t = en >> 2, va = en & 3;
+                if (t >= h.n_elem || t <= last || verts[t * 4 + va] != ra) return MF_ERR_STATE;
+                last = t;
+                for (int k = 0; k < 4; k++) holds[verts[t * 4 + k]] |= 1ull << i;
+            }
+            if (cnt[ra] != val) return MF_ERR_STATE;
+            // the row's entries: one per CSR position, each mask exact
+            for (int o = 0; o < len; o++) {
+                const size_t e = (size_t)h.ent_start + ri.ent + o;
+                const int era = p.ent_info[e] & 0xFF, ec = p.ent_info[e] >> 8;
+                if (era != ra || ec >= h.n_node || nodes[ec] != col[row[a] + o] - 1) return MF_ERR_STATE;
+                if (hit[(size_t)row[a] + o]++) return MF_ERR_STATE;
+                uint64_t m = p.ent_mask[e * MW];
+                if (MW == 2) m |= (uint64_t)p.ent_mask[e * MW + 1] << 32;
+                const uint64_t want = ec == ra ? 0 : holds[ec];
+                if (m != want || (ec != ra && m == 0)) return MF_ERR_STATE;
+            }
+            for (int i = 0; i < val; i++) {
+                const int t = p.inc[(size_t)h.inc_start + incOff + i] >> 2;
+                for (int k = 0; k < 4; k++) holds[verts[t * 4 + k]] = 0;
             }
         }
-        for (int ra = 0; ra < h.n_rows; ra++) {
-            const int a = p.tile_nodes[h.node_start + ra];
-            if ((row[a + 1] > row[a]) != (diagSeen[ra] == 1)) return MF_ERR_STATE;
-        }
+        if (entNext != h.n_ent || incNext > h.n_inc || h.n_inc - incNext >= 8) return MF_ERR_STATE;
+        for (int i = h.inc_start; i < h.inc_start + h.n_inc; i++)
+            if ((p.inc[i] >> 2) >= std::max(h.n_elem, 1)) return MF_ERR_STATE;  // padding too
     }
     for (int i = 0; i < p.N; i++)
         if (!rowSeen[i]) return MF_ERR_STATE;

@@ -1,19 +1,19 @@
 // Tiled gather schedule for one mesh topology (host-built, device-resident).
 //
-// Rows (nodes) are grouped into tiles by recursive coordinate bisection. A tile's
-// workgroup
-//   A)  loads the coordinates of every node its elements touch into LDS, then computes
-//       the scaled gradients (and coefficients) of every element incident to one of its
-//       rows, once per tile, into LDS;
-//   B1) "diagonal chunks": one lane per tile row sums the row's diagonal block over all
-//       its incident elements, in ascending (color-permuted) element order -- the
-//       summation order of the reference's COLORING build -- keeps it in LDS and inverts
-//       it into the block-Jacobi prec (prec_init + prec_inversion fused);
-//   B2) "row chunks": whole rows packed into 64-lane waves, one lane per CSR entry in
-//       CSR order; an off-diagonal lane sums the elements containing both its row and
-//       its column node (same order), the diagonal lane copies its block from LDS, and
-//       the wave writes complete row slices, consecutive lanes on consecutive blocks.
-// Every CSR entry is written exactly once; values need no zero-fill.
+// The nodes that own a CSR row are grouped into tiles by recursive coordinate bisection.
+// A tile's workgroup
+//   A) loads the coordinates of every node its elements touch into LDS and computes the
+//      scaled gradients and coefficients of every element incident to one of its rows,
+//      once per tile, into an LDS element stage;
+//   B) sums every CSR block of its rows from the stage, each block in ascending
+//      (color-permuted) element order -- the summation order of the reference's
+//      COLORING build, src/assembly.cc:246-271 -- and writes it once:
+//      - a row's incident elements are kept in LDS as an ascending "incidence list";
+//      - an off-diagonal block (row a, column c) is one lane, which walks the bits of a
+//        precomputed mask over row a's incidence list (the elements that also hold c);
+//      - a diagonal block (the whole incidence list) is one lane, which also inverts
+//        it into the block-Jacobi prec (prec_init + prec_inversion fused).
+// Every CSR entry and prec block is written exactly once; values need no zero-fill.
 #pragma once
 
 #include <cstdint>
@@ -28,103 +28,112 @@
 
 namespace mf {
 
-constexpr int kTaskLanes = 64;           // tasks per chunk = lanes per wave
-constexpr int kDiagGroup = 4;            // diagonal-chunk contributions per 8-byte load
-constexpr int kRowGroup = 2;             // row-chunk contributions per 4-byte load
-constexpr int kRowPrefetch = 4;          // row-chunk groups loaded before the stores
-constexpr uint32_t kNoTask = 0xFFFFFFFFu;
-constexpr uint32_t kDiagLane = 1u << 16; // row-chunk task flag: copy the diagonal block
-constexpr int kMaxTileElems = 4094;      // contribution entries carry a 12-bit element index
-constexpr int kMaxTileNodes = 256;       // tile-local vertex ids are 8-bit
-constexpr int kMaxTileRows = 256;        // task words carry 8-bit rows
-constexpr int kMaxRowLen = 256;          // and 8-bit offsets inside a row
+constexpr int kLanes = 64;               // lanes per wave = CSR blocks per work block
+constexpr int kMaxTileElems = 2048;      // incidence entries carry an element index < 2^14
+constexpr int kMaxTileNodes = 256;       // tile-local node ids are 8-bit
+constexpr int kMaxTileRows = 256;        // tile-local row ids are 8-bit
+constexpr int kMaxTileInc = 8192;        // incidence entries per tile (16-bit offsets)
+constexpr int kMaxTileBlocks = 32;       // work blocks (diagonal + entry) per tile
+constexpr int kMaxValence = 64;          // incident elements per row (mask width)
 constexpr int kMaxLdsBytes = 160 * 1024; // LDS of one gfx950 CU
+constexpr int kIncAlign = 4;             // row incidence lists start at multiples of 4
 
-// Element-stage slot of one tile element, in units of the stage type (double or
-// float): the (x,y) gradient pairs of vertices 0..3 at 2i (16-byte aligned for
-// double2 reads), the z components at 8+i, then the coefficients at 12.. (ela:
-// lambda+2mu, lambda+mu, mu; lap: k), padded to an even count.
+// Element-stage slot of one tile element, in units of the stage type (double or float):
+// the (x,y) gradient pairs of vertices 0..3 at 2i (16-byte aligned for double2 reads),
+// the z components at 8+i, then the coefficients at 12.. (ela: lambda+2mu, lambda+mu,
+// mu; lap: k). The slot length is an odd number of pairs (an odd multiple of 16 bytes
+// in double, of 8 bytes in float): consecutive elements then start in different LDS
+// bank groups, so the per-element stage writes are conflict-free and the gathers of
+// scattered elements spread over all banks.
 constexpr int kStageEla = 18;
 constexpr int kStageLap = 14;
 
-// LDS of one tile, in this order (offsets 16-byte aligned where a 16-byte access needs
-// it): [element stage: n_elem+1 slots, the last one all zero and the target of padding
-// contributions] [diagonal blocks: n_rows] [prec blocks: n_rows] [node coordinates:
-// n_node*24] [row offsets, row lengths, Dirichlet dof masks: n_rows*4 each].
+MF_HOST_DEVICE inline int align16(int x) { return (x + 15) & ~15; }
+
+// LDS of one tile: [element stage: n_elem slots] [row info: n_rows x 16 B] [node
+// coordinates: n_node x 24 B] [row node ids: n_rows x 4 B] [element vertices: n_elem x
+// 4 B] [incidence lists: n_inc x 2 B, n_inc a multiple of 8].
 struct TileLds {
-    int stage, diag, prec, coords, rowptr, rowlen, mask, total;
+    int stage, rinfo, coords, rnode, verts, inc, total;
 };
 
-MF_HOST_DEVICE inline TileLds tile_lds_layout(int n_node, int n_elem, int n_rows,
-                                              int slot_bytes, int block_bytes)
+MF_HOST_DEVICE inline TileLds tile_lds_layout(int n_node, int n_elem, int n_rows, int n_inc,
+                                              int slot_bytes)
 {
     TileLds L;
     L.stage = 0;
-    L.diag = ((n_elem + 1) * slot_bytes + 15) & ~15;
-    L.prec = L.diag + n_rows * block_bytes;
-    L.coords = (L.prec + n_rows * block_bytes + 15) & ~15;
-    L.rowptr = L.coords + n_node * 24;
-    L.rowlen = L.rowptr + n_rows * 4;
-    L.mask = L.rowlen + n_rows * 4;
-    L.total = L.mask + n_rows * 4;
+    L.rinfo = align16(n_elem * slot_bytes);
+    L.coords = L.rinfo + n_rows * 16;
+    L.rnode = L.coords + n_node * 24;
+    L.verts = L.rnode + n_rows * 4;
+    L.inc = align16(L.verts + n_elem * 4);
+    L.total = L.inc + align16(n_inc * 2);
     return L;
 }
 
+// Diagonal work blocks of a tile: one row per lane.
+MF_HOST_DEVICE inline int diag_blocks(bool, int n_rows) { return (n_rows + kLanes - 1) / kLanes; }
+
 struct TileHdr {
-    int32_t elem_start, n_elem;      // into tile_elems / tile_verts
-    int32_t node_start, n_node;      // into tile_nodes; the first n_rows are the tile's rows
+    int32_t elem_start, n_elem;  // into tile_elems / tile_verts (ascending element ids)
+    int32_t node_start, n_node;  // into tile_nodes; the first n_rows are the tile's rows
     int32_t n_rows;
-    int32_t dchunk_start, n_dchunks; // diagonal chunks (into chunks; tasks at [c*64 ...])
-    int32_t rchunk_start, n_rchunks; // row chunks
-    int32_t rinfo_start;             // into tile_rowinfo: n_rows entries
+    int32_t rinfo_start;         // into tile_rowinfo: n_rows entries
+    int32_t inc_start, n_inc;    // into inc (multiples of 8)
+    int32_t ent_start, n_ent;    // into ent_mask / ent_info: the tile rows' CSR entries
     int32_t pad[2];
 };
 
-// Static per-tile-row data: the row's CSR offset and (length | Dirichlet mask << 16),
-// mask bit k set when dof k of the node is constrained (ela only).
+// Static per-tile-row data (int4 in LDS):
+//   rowptr:   CSR position of the row's first entry;
+//   len_diag: row length | diagonal offset << 16 | Dirichlet mask << 24 (bit k set when
+//             dof k of the node is constrained, ela only);
+//   inc:      tile-local offset of the row's incidence list | valence << 16;
+//   ent:      tile-local index of the row's first entry.
 struct RowInfo {
     int32_t rowptr;
-    int32_t len_mask;
+    int32_t len_diag;
+    int32_t inc;
+    int32_t ent;
 };
 
-// Contributions of a chunk are stored in groups of G steps (G = kDiagGroup for diagonal
-// chunks, kRowGroup for row chunks): group g is 64 lanes x G entries, lane l's entries
-// contiguous, so one load per lane fetches its next G contributions and a wave's load is
-// one contiguous block. Step s of lane l is at contrib_start + (s/G)*64G + l*G + s%G.
-// An entry is t<<4 | a<<2 | k (tile element t, its local vertices of the row and of the
-// column node); padding entries are n_elem<<4, the all-zero slot.
-// Diagonal-chunk task word: the tile row. Row-chunk task word: ra | oa<<8 (| kDiagLane
-// for the row's diagonal entry), oa the entry's offset inside row ra.
-struct ChunkHdr {
-    int32_t contrib_start;
-    int32_t steps;
-};
-
+// Incidence entry: t << 2 | a, t the tile element, a the row node's vertex slot in it.
+// Each row's list starts at a multiple of kIncAlign entries (zero padding between lists:
+// element 0, read but never summed), so a lane reads 4 entries with one 8-byte access.
+// Entries (CSR blocks) of a tile are stored in tile-row order, CSR order inside a row:
+//   ent_info[e] = ra | c << 8  (tile row, tile-local column node; c == ra on the diagonal)
+//   ent_mask[e * mask_words + w]: bit b set when the b-th element of row ra's
+//   incidence list holds node c (zero on the diagonal).
 struct PlanHost {
-    int op = 0, nE = 0, N = 0, nnz = 0;
+    int op = 0, nE = 0, N = 0, nnz = 0, block = 256, mask_words = 1;
     std::vector<TileHdr> tiles;
     std::vector<int32_t> tile_elems;   // global (permuted) element ids, ascending per tile
     std::vector<uint8_t> tile_verts;   // 4 tile-local node ids per tile element
     std::vector<int32_t> tile_nodes;   // global node ids (0-based)
     std::vector<RowInfo> tile_rowinfo; // per tile row
-    std::vector<ChunkHdr> chunks;
-    std::vector<uint32_t> tasks;       // 64 task words per chunk
-    std::vector<uint16_t> contribs;
+    std::vector<uint16_t> inc;
+    std::vector<uint32_t> ent_mask;
+    std::vector<uint16_t> ent_info;
     std::vector<int32_t> isolated;     // nodes with an empty row (no element)
     int max_node = 0, max_elem = 0, max_rows = 0;
-    int64_t total_tasks = 0;
 };
 
 // Builds the schedule. coord (3N doubles) may be null: nodes are then ordered by a
-// breadth-first sweep of the node graph instead of coordinate bisection.
-// slot_bytes: LDS bytes of one element-stage slot of the widest variant that will run
-// it; tiles are grown while their LDS layout stays <= lds_budget.
+// breadth-first sweep of the node graph instead of coordinate bisection. slot_bytes:
+// LDS bytes of one element-stage slot of the widest variant that will run it; tiles are
+// grown while their LDS layout stays <= lds_budget and the kernel's per-thread limits
+// for a workgroup of `block` threads hold. checkBounds (3N, ela) may be null.
 int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, const int *row,
                const int *col, const int *checkBounds, const double *coord, int lds_budget,
-               int max_rows, int slot_bytes);
+               int max_rows, int slot_bytes, int block);
 
-// Checks every index the kernel will follow against the array it indexes and that every
-// CSR entry is covered exactly once. Returns 0 or a negative status.
-int validate_plan(const PlanHost &p, const int *row, const int *col, int slot_bytes);
+// Checks every index the kernel will follow against the array it indexes, that every
+// CSR entry is covered exactly once and that every entry's mask selects exactly the
+// elements holding both its nodes. Returns 0 or a negative status.
+int validate_plan(const PlanHost &p, const int *row, const int *col, const int *e2n,
+                  int slot_bytes);
+
+// Per-thread register limits of the kernel for a workgroup of `block` threads.
+inline int elems_per_thread(int block) { return kMaxTileElems / block; }
 
 }  // namespace mf

@@ -218,7 +218,7 @@ class Plan:
     def stats(self):
         out = np.zeros(8, np.int64)
         check(self._lib.mf_plan_stats(self.ptr, out.ctypes.data), "mf_plan_stats")
-        keys = ["tiles", "tile_elems", "tasks", "contribs", "meta_bytes", "lds_bytes",
+        keys = ["tiles", "tile_elems", "entries", "incidences", "plan_bytes", "lds_bytes",
                 "block_threads", "max_rows"]
         return dict(zip(keys, (int(x) for x in out)))
 

@@ -213,6 +213,6 @@ def plan_host_stats(mesh, tile_lds_bytes=0, block_threads=0, max_tile_rows=0, us
                                    _ptr(mesh.elemToNode), _ptr(mesh.row), _ptr(mesh.col),
                                    _ptr(mesh.coord) if use_coord else None,
                                    ctypes.byref(params), out.ctypes.data), "mf_plan_host_stats")
-    keys = ["tiles", "tile_elems", "tasks", "contribs", "meta_bytes", "lds_bytes",
+    keys = ["tiles", "tile_elems", "entries", "incidences", "plan_bytes", "lds_bytes",
             "block_threads", "max_rows"]
     return dict(zip(keys, (int(x) for x in out)))

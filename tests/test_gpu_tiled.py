@@ -42,7 +42,7 @@ def test_tiled_bit_exact_30_cube(mf, orc, gpu, op, coef_seed):
 
 @pytest.mark.parametrize("op", [0, 1])
 @pytest.mark.parametrize("params", [
-    dict(block_threads=128), dict(block_threads=512), dict(tile_lds_bytes=8192),
+    dict(block_threads=256), dict(block_threads=512), dict(tile_lds_bytes=8192),
     dict(tile_lds_bytes=150 * 1024, max_tile_rows=254), dict(use_coord=False),
     dict(max_tile_rows=1),
 ])
@@ -50,6 +50,35 @@ def test_tiled_schedule_variants(mf, orc, gpu, op, params):
     print(f"[tiled variant] op={op} {params}", flush=True)
     mesh, om = build(mf, orc, 12, op, jitter=0.25, seed=3, shuffle=203, coef_seed=9)
     plan, dm = run_plan(gpu, mesh, **params)
+    ov = om.assemble()
+    assert np.array_equal(dm.values.numpy(), ov)
+    assert np.array_equal(dm.prec.numpy(), om.prec(ov)[0])
+
+
+def fan_mesh(mf, orc, n_fan, op, coef_seed=5):
+    """n_fan tets around the edge between two hub nodes: the hubs have valence n_fan."""
+    ang = np.linspace(0, 2 * np.pi, n_fan + 1)[:-1]
+    ring = np.stack([np.cos(ang), np.sin(ang), 0.5 + 0.1 * np.sin(3 * ang)], 1)
+    coord = np.concatenate([[[0, 0, 0], [0.05, -0.02, 1]], ring]).astype(np.float64).ravel()
+    e2n = []
+    for i in range(n_fan):
+        e2n += [1, 2, 3 + i, 3 + (i + 1) % n_fan]
+    e2n = np.array(e2n, np.int32)
+    code = np.zeros(n_fan + 2, np.int32)
+    code[5] = 52
+    nnz = 2 * (n_fan + 2) + 5 * n_fan
+    coef = mf.generate_coef(op, n_fan, coef_seed)
+    return (mf.Mesh(coord, e2n, code, nnz, op, coef=coef),
+            orc.OracleMesh(coord, e2n, code, nnz, op, coef=coef))
+
+
+@pytest.mark.parametrize("op", [0, 1])
+@pytest.mark.parametrize("n_fan", [24, 40, 64])
+def test_tiled_high_valence_masks(mf, orc, gpu, op, n_fan):
+    """Valence up to 32 uses one mask word per entry, 33..64 two: both bit-exact."""
+    print(f"[tiled fan] op={op} n_fan={n_fan}", flush=True)
+    mesh, om = fan_mesh(mf, orc, n_fan, op)
+    plan, dm = run_plan(gpu, mesh)
     ov = om.assemble()
     assert np.array_equal(dm.values.numpy(), ov)
     assert np.array_equal(dm.prec.numpy(), om.prec(ov)[0])

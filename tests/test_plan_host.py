@@ -1,6 +1,6 @@
 """The tiled schedule's host builder and validator (no GPU): every CSR entry is covered by
-exactly one task, every index the kernel follows is in bounds, on several meshes and
-tuning parameters; invalid inputs are rejected."""
+exactly one entry lane with an exact element mask, every index the kernel follows is in
+bounds, on several meshes and tuning parameters; invalid inputs are rejected."""
 import numpy as np
 import pytest
 
@@ -11,7 +11,7 @@ from minifem.setup import plan_host_stats
 @pytest.mark.parametrize("op", [0, 1])
 @pytest.mark.parametrize("params", [
     dict(), dict(tile_lds_bytes=8192), dict(tile_lds_bytes=150 * 1024, max_tile_rows=254),
-    dict(max_tile_rows=1), dict(use_coord=False),
+    dict(max_tile_rows=1), dict(use_coord=False), dict(block_threads=512),
 ])
 def test_plan_builds_and_validates(mf, n, shuffle, op, params):
     coord, e2n, code, nnz = mf.generate_cube(n, 0.2, 3, shuffle)
@@ -19,19 +19,20 @@ def test_plan_builds_and_validates(mf, n, shuffle, op, params):
     s = plan_host_stats(mesh, **params)
     assert s["tiles"] >= 1
     assert s["tile_elems"] >= mesh.nbElem  # every element is staged at least once
-    # one diagonal task per non-empty row plus one row-chunk lane per CSR entry
-    n_rows = int(np.count_nonzero(np.diff(mesh.row)))
-    assert s["tasks"] == n_rows + mesh.nnz
-    # every (row, element, vertex) incidence is one contribution, diagonals twice
-    # (their own sum and the zero-contribution row lane is padding)
-    assert s["contribs"] >= 16 * mesh.nbElem
+    assert s["entries"] == mesh.nnz        # one entry lane per CSR block
+    # every (row, incident element) pair once; each row's list padded to 4, each tile's
+    # to 8
+    assert 4 * mesh.nbElem <= s["incidences"] <= 4 * mesh.nbElem + 3 * mesh.nbNodes + 4 * s["tiles"]
+    assert s["incidences"] % 4 == 0
+    if "max_tile_rows" in params:
+        assert s["max_rows"] <= params["max_tile_rows"]
 
 
 def test_plan_rejects_bad_parameters(mf):
     coord, e2n, code, nnz = mf.generate_cube(4, 0.2, 3, 0)
     mesh = mf.Mesh(coord, e2n, code, nnz, 1)
     for bad in (dict(tile_lds_bytes=100), dict(tile_lds_bytes=10 ** 6),
-                dict(max_tile_rows=300)):
+                dict(max_tile_rows=300), dict(block_threads=128), dict(block_threads=384)):
         with pytest.raises(mf.MiniFemError):
             plan_host_stats(mesh, **bad)
 
@@ -43,3 +44,39 @@ def test_plan_rejects_inconsistent_pattern(mf):
     mesh.row[-1] -= 1  # row_ptr no longer ends at nnz
     with pytest.raises(mf.MiniFemError):
         plan_host_stats(mesh)
+
+
+def test_plan_rejects_valence_above_64(mf):
+    """A node in more than 64 elements does not fit the 64-bit entry masks: the plan is
+    refused (the colored schedule handles such meshes)."""
+    # a fan of 70 tets around the edge (0, 1): nodes 0, 1 in every element
+    n_fan = 70
+    ang = np.linspace(0, 2 * np.pi, n_fan + 1)[:-1]
+    ring = np.stack([np.cos(ang), np.sin(ang), np.full(n_fan, 0.5)], 1)
+    coord = np.concatenate([[[0, 0, 0], [0, 0, 1]], ring]).astype(np.float64)
+    e2n = []
+    for i in range(n_fan):
+        e2n += [1, 2, 3 + i, 3 + (i + 1) % n_fan]
+    e2n = np.array(e2n, np.int32)
+    code = np.zeros(coord.shape[0], np.int32)
+    nnz = 2 * (n_fan + 2) + 5 * n_fan  # hub rows see every node, ring rows 5
+    mesh = mf.Mesh(coord.ravel(), e2n, code, nnz, 0)
+    with pytest.raises(mf.MiniFemError):
+        plan_host_stats(mesh)
+
+
+def test_plan_accepts_valence_between_32_and_64(mf):
+    """Valence 33..64 selects the two-word entry masks."""
+    n_fan = 40
+    ang = np.linspace(0, 2 * np.pi, n_fan + 1)[:-1]
+    ring = np.stack([np.cos(ang), np.sin(ang), np.full(n_fan, 0.5)], 1)
+    coord = np.concatenate([[[0, 0, 0], [0, 0, 1]], ring]).astype(np.float64)
+    e2n = []
+    for i in range(n_fan):
+        e2n += [1, 2, 3 + i, 3 + (i + 1) % n_fan]
+    e2n = np.array(e2n, np.int32)
+    code = np.zeros(coord.shape[0], np.int32)
+    nnz = 2 * (n_fan + 2) + 5 * n_fan
+    mesh = mf.Mesh(coord.ravel(), e2n, code, nnz, 0)
+    s = plan_host_stats(mesh)
+    assert s["entries"] == mesh.nnz
