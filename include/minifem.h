@@ -205,18 +205,23 @@ int mf_norm(const double *x, int64_t n, double *result, void *stream);
 
 typedef struct mf_plan mf_plan;
 
-/* Tuning knobs of the tiled schedule (0 = default). */
+/* Tuning knobs of the tiled schedule (0 = default). An off-diagonal block (c,a) is
+ * the exact transpose of (a,c) (same elements, same order, commutative products), so
+ * the plan computes each pair once and writes both: always when a and c are rows of
+ * one tile; across tiles that run on the same XCD at most pair_window dispatch slots
+ * apart when pair_window > 0 (< 0: within tiles only). */
 typedef struct mf_plan_params {
-    int tile_lds_bytes;   /* LDS budget of one tile's element stage */
-    int block_threads;    /* threads per workgroup (multiple of 64) */
-    int max_tile_rows;    /* cap on rows per tile */
+    int tile_lds_bytes;   /* LDS budget of one tile: 4096..163840 (default 65536) */
+    int block_threads;    /* threads per workgroup: 512 or 1024 (default 512) */
+    int max_tile_rows;    /* cap on rows per tile, <= 256 (default 64 ela, 128 lap) */
+    int pair_window;      /* cross-tile pairing window, see above */
 } mf_plan_params;
 
 /* Host arrays: elemToNode (color-permuted, 1-based, 4nE), row[N+1], col[nnz],
  * checkBounds[3N] (may be NULL), coord[3N] (may be NULL; used only to group nodes into
  * tiles by coordinate bisection, else a breadth-first node order is used). The plan
- * copies what it needs to the GPU and checks every index its kernel follows. Limits:
- * rows of at most 256 entries, at most 4095 elements incident to one tile. */
+ * copies what it needs to the GPU and checks every index its kernel follows. Limit: at
+ * most 64 elements per node (MF_ERR_ARG otherwise; the colored path has none). */
 int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nbNodes, int nnz,
                    const int *elemToNode, const int *row, const int *col,
                    const int *checkBounds, const double *coord,
@@ -229,9 +234,10 @@ int mf_plan_host_stats(int operatorID, int nbElem, int nbNodes, int nnz,
                        const int *elemToNode, const int *row, const int *col,
                        const double *coord, const mf_plan_params *params, int64_t out[8]);
 
-/* Plan statistics: out[0] tiles, [1] tile-element slots, [2] tasks, [3]
- * contributions, [4] metadata bytes on the device, [5] LDS bytes per workgroup,
- * [6] threads per workgroup, [7] rows per tile (max). */
+/* Plan statistics: out[0] tiles, [1] tile-element slots, [2] off-diagonal tasks
+ * (a pair computed once counts once), [3] incidence entries, [4] plan bytes on the
+ * device, [5] LDS bytes per workgroup, [6] threads per workgroup, [7] rows per tile
+ * (max). */
 int mf_plan_stats(const mf_plan *plan, int64_t out[8]);
 
 /* One iteration: values (assembly, src/FEM.cc:183) and the inverted block-Jacobi prec

@@ -26,15 +26,17 @@ struct mf_plan {
     int32_t *tile_nodes = nullptr;
     int4 *tile_rowinfo = nullptr;
     uint16_t *inc = nullptr;
-    uint32_t *ent_mask = nullptr;
-    uint16_t *ent_info = nullptr;
+    uint32_t *task_mask = nullptr;
+    uint32_t *task_info = nullptr;
+    int32_t *task_xpos = nullptr;
     int32_t *cb = nullptr;
     int32_t *isolated = nullptr;
 };
 
 namespace {
 
-constexpr int kXcd = 8;  // MI355X: workgroups are dealt to the 8 XCDs round-robin
+constexpr int kXcd = mf::kXcds;
+constexpr int kDefaultPairWindow = 0;  // cross-tile pairing off unless asked for
 constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
 
 template <typename T>
@@ -76,22 +78,25 @@ __device__ __forceinline__ void stage_grad(const T *sl, int v, T (&g)[3])
     g[2] = sl[8 + v];
 }
 
-// One tile is one workgroup. blockIdx -> tile is XCD-aware: XCD x (= blockIdx % 8) walks
-// the x-th contiguous eighth of the tiles in partition order, so tiles that share
-// elements and nodes run on the same XCD close in time and share its L2.
+// Persistent workgroups, XCD-aware. The tiles are cut into 8 contiguous ranges in
+// partition order, range x for XCD x (workgroup b runs on XCD b % 8); the G workgroups of
+// an XCD walk its range together, slot s0, s0 + G, ..., so tiles that share elements and
+// nodes run on one XCD close in time and share its L2.
 //
-// Global loads come in two dependent rounds: (1) everything static the tile needs --
-// node and element ids, vertex lists, row info, incidence lists and this wave's entry
-// masks; (2) the coordinate and coefficient gathers. Phase B then reads only LDS and
-// registers, so its stores are never followed by a load (gfx950's vmcnt retires loads
-// and stores in order).
+// Per tile: fill LDS from registers -> A (element stage) -> B (blocks, stores). The next
+// tile's global loads overlap the current tile: its static data (ids, vertex lists, row
+// info, incidence lists, task words) is issued after A, its coordinate and coefficient
+// gathers after the first block of B has been computed and before that block's stores
+// (gfx950's vmcnt retires loads and stores in order, so a load issued behind stores waits
+// for them).
 template <int OP, int PREC, int BT, int MW>
-__global__ void __launch_bounds__(BT)
-    k_tiled(const mf::TileHdr *__restrict__ tiles, int ntiles,
+__global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
+    k_tiled(const mf::TileHdr *__restrict__ tiles, int ntiles, int G,
             const int32_t *__restrict__ tile_elems, const uint32_t *__restrict__ tile_verts,
             const int32_t *__restrict__ tile_nodes, const int4 *__restrict__ rowinfo,
-            const uint4 *__restrict__ inc8, const uint32_t *__restrict__ ent_mask,
-            const uint16_t *__restrict__ ent_info, const double *__restrict__ coord,
+            const uint4 *__restrict__ inc8, const uint32_t *__restrict__ task_mask,
+            const uint32_t *__restrict__ task_info, const int32_t *__restrict__ task_xpos,
+            const double *__restrict__ coord,
             const double *__restrict__ coef, double *__restrict__ values,
             double *__restrict__ prec, int *__restrict__ singular)
 {
@@ -102,177 +107,198 @@ __global__ void __launch_bounds__(BT)
     constexpr int S = ELA ? mf::kStageEla : mf::kStageLap;
     constexpr int NA = ELA ? 9 : 1;
     constexpr int NW = BT / 64;
-    constexpr int EPT = mf::kMaxTileElems / BT;                // tile elements per thread
+    constexpr int EPT = 1;                                     // tile elements per thread
     constexpr int NPT = (mf::kMaxTileNodes + BT - 1) / BT;     // tile nodes per thread
     constexpr int IPT = (mf::kMaxTileInc / 8 + BT - 1) / BT;   // 16-byte incidence loads
-    constexpr int PB = mf::kMaxTileBlocks / NW;                // work blocks per wave
+    constexpr int PB = mf::kMaxTileBlocks / NW;                // task blocks per wave
     extern __shared__ __align__(16) unsigned char lds_raw[];
 
     const int per = (ntiles + kXcd - 1) / kXcd;
-    const int xcd = blockIdx.x % kXcd, slot = blockIdx.x / kXcd;
-    const int tile = xcd * per + slot;
-    if (slot >= per || tile >= ntiles) return;
-    const mf::TileHdr h = tiles[tile];
-    const int NN = h.n_node, TE = h.n_elem, R = h.n_rows;
-    const mf::TileLds L = mf::tile_lds_layout(NN, TE, R, h.n_inc, S * (int)sizeof(T));
-    T *stage = reinterpret_cast<T *>(lds_raw + L.stage);
-    int4 *lri = reinterpret_cast<int4 *>(lds_raw + L.rinfo);
-    double *lc = reinterpret_cast<double *>(lds_raw + L.coords);
-    int *lrnode = reinterpret_cast<int *>(lds_raw + L.rnode);
-    uint32_t *lverts = reinterpret_cast<uint32_t *>(lds_raw + L.verts);
-    uint16_t *linc = reinterpret_cast<uint16_t *>(lds_raw + L.inc);
+    const int xcd = blockIdx.x % kXcd;
+    int slot = blockIdx.x / kXcd;
+    if (slot >= per || xcd * per + slot >= ntiles) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // Work blocks: diagonal block w (rows 64w..) on wave w; entry blocks m (CSR entries
-    // 64m..) on the other waves while they hold at most 4 each (a diagonal block costs
-    // about as much as four entry blocks), else on all waves round-robin.
-    const int nDB = mf::diag_blocks(ELA, R);
-    const int nEB = (h.n_ent + 63) / 64;
-    const bool spread = nEB > 4 * (NW - nDB);
-    const int m0 = spread ? wave : wave - nDB, mstep = spread ? NW : NW - nDB;
 
-    // ---- round 1: static tile data
-    int nodeId[NPT];
-#pragma unroll
-    for (int r = 0; r < NPT; r++) {
-        const int i = tid + r * BT;
-        nodeId[r] = i < NN ? tile_nodes[h.node_start + i] : 0;
-    }
-    int elemId[EPT];
+    // registers holding one tile's global data: static (round 1) and gathered (round 2)
+    int nodeId[NPT], elemId[EPT];
     uint32_t vert[EPT];
-#pragma unroll
-    for (int r = 0; r < EPT; r++) {
-        const int t = tid + r * BT;
-        elemId[r] = t < TE ? tile_elems[h.elem_start + t] : 0;
-        vert[r] = t < TE ? tile_verts[h.elem_start + t] : 0u;
-    }
     int4 rinf[NPT];
-#pragma unroll
-    for (int r = 0; r < NPT; r++) {
-        const int i = tid + r * BT;
-        rinf[r] = i < R ? rowinfo[h.rinfo_start + i] : make_int4(0, 0, 0, 0);
-    }
     uint4 incv[IPT];
-#pragma unroll
-    for (int r = 0; r < IPT; r++) {
-        const int i = tid + r * BT;
-        incv[r] = i < h.n_inc / 8 ? inc8[h.inc_start / 8 + i] : make_uint4(0, 0, 0, 0);
-    }
     uint32_t pinf[PB];
+    int32_t pxp[PB];
     M pm[PB];
+    double cx[NPT][3], kc0[EPT], kc1[EPT];
+
+    auto load_static = [&](const mf::TileHdr &hh) {
 #pragma unroll
-    for (int k = 0; k < PB; k++) {
-        const int idx = (m0 + k * mstep) * 64 + lane;
-        pinf[k] = kNoEntry;
-        pm[k] = 0;
-        if (m0 >= 0 && idx < h.n_ent) {
-            const int64_t e = (int64_t)h.ent_start + idx;
-            pinf[k] = ent_info[e];
-            if constexpr (MW == 1) {
-                pm[k] = ent_mask[e];
-            } else {
-                const uint2 w = reinterpret_cast<const uint2 *>(ent_mask)[e];
-                pm[k] = (uint64_t)w.x | (uint64_t)w.y << 32;
+        for (int r = 0; r < NPT; r++) {
+            const int i = tid + r * BT;
+            nodeId[r] = i < hh.n_node ? tile_nodes[hh.node_start + i] : 0;
+            rinf[r] = i < hh.n_rows ? rowinfo[hh.rinfo_start + i] : make_int4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < EPT; r++) {
+            const int t = tid + r * BT;
+            elemId[r] = t < hh.n_elem ? tile_elems[hh.elem_start + t] : 0;
+            vert[r] = t < hh.n_elem ? tile_verts[hh.elem_start + t] : 0u;
+        }
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            const int i = tid + r * BT;
+            incv[r] = i < hh.n_inc / 8 ? inc8[hh.inc_start / 8 + i] : make_uint4(0, 0, 0, 0);
+        }
+        int dg, m0, mstep;
+        mf::task_blocks(hh.n_rows, hh.n_task, NW, wave, dg, m0, mstep);
+#pragma unroll
+        for (int k = 0; k < PB; k++) {
+            const int idx = (m0 + k * mstep) * 64 + lane;
+            pinf[k] = kNoEntry;
+            pxp[k] = -1;
+            pm[k] = 0;
+            if (m0 >= 0 && idx < hh.n_task) {
+                const int64_t e = (int64_t)hh.task_start + idx;
+                pinf[k] = task_info[e];
+                pxp[k] = task_xpos[e];
+                if constexpr (MW == 1) {
+                    pm[k] = task_mask[e];
+                } else {
+                    const uint2 w = reinterpret_cast<const uint2 *>(task_mask)[e];
+                    pm[k] = (uint64_t)w.x | (uint64_t)w.y << 32;
+                }
             }
         }
-    }
-
-    // ---- round 2: coordinate and coefficient gathers
-    double cx[NPT][3];
+    };
+    auto load_gathers = [&](const mf::TileHdr &hh) {
 #pragma unroll
-    for (int r = 0; r < NPT; r++) {
-        const int i = tid + r * BT;
-        cx[r][0] = cx[r][1] = cx[r][2] = 0.0;
-        if (i < NN) {
-            const double *p = coord + (int64_t)nodeId[r] * 3;
-            cx[r][0] = p[0];
-            cx[r][1] = p[1];
-            cx[r][2] = p[2];
+        for (int r = 0; r < NPT; r++) {
+            const int i = tid + r * BT;
+            cx[r][0] = cx[r][1] = cx[r][2] = 0.0;
+            if (i < hh.n_node) {
+                const double *p = coord + (int64_t)nodeId[r] * 3;
+                cx[r][0] = p[0];
+                cx[r][1] = p[1];
+                cx[r][2] = p[2];
+            }
         }
-    }
-    double kc0[EPT], kc1[EPT];
 #pragma unroll
-    for (int r = 0; r < EPT; r++) {
-        const int t = tid + r * BT;
-        kc0[r] = ELA ? 0.25 : 1.0;  // reference constants: lambda 0.25, mu 1 | k 1
-        kc1[r] = 1.0;
-        if (t < TE && coef) {
+        for (int r = 0; r < EPT; r++) {
+            const int t = tid + r * BT;
+            kc0[r] = ELA ? 0.25 : 1.0;  // reference constants: lambda 0.25, mu 1 | k 1
+            kc1[r] = 1.0;
+            if (t < hh.n_elem && coef) {
+                if constexpr (ELA) {
+                    const double2 lm = reinterpret_cast<const double2 *>(coef)[elemId[r]];
+                    kc0[r] = lm.x;
+                    kc1[r] = lm.y;
+                } else {
+                    kc0[r] = coef[elemId[r]];
+                }
+            }
+        }
+    };
+
+    mf::TileHdr h = tiles[xcd * per + slot];
+    load_static(h);
+    load_gathers(h);
+    for (;;) {
+        const int NN = h.n_node, TE = h.n_elem, R = h.n_rows;
+        const mf::TileLds L = mf::tile_lds_layout(NN, TE, R, h.n_inc, S * (int)sizeof(T));
+        T *stage = reinterpret_cast<T *>(lds_raw + L.stage);
+        int4 *lri = reinterpret_cast<int4 *>(lds_raw + L.rinfo);
+        double *lc = reinterpret_cast<double *>(lds_raw + L.coords);
+        int *lrnode = reinterpret_cast<int *>(lds_raw + L.rnode);
+        uint32_t *lverts = reinterpret_cast<uint32_t *>(lds_raw + L.verts);
+        uint16_t *linc = reinterpret_cast<uint16_t *>(lds_raw + L.inc);
+
+        // ---- LDS fill
+#pragma unroll
+        for (int r = 0; r < NPT; r++) {
+            const int i = tid + r * BT;
+            if (i < R) {
+                lri[i] = rinf[r];
+                lrnode[i] = nodeId[r];
+            }
+            if (i < NN) {
+                lc[i * 3 + 0] = cx[r][0];
+                lc[i * 3 + 1] = cx[r][1];
+                lc[i * 3 + 2] = cx[r][2];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            const int i = tid + r * BT;
+            if (i < h.n_inc / 8) reinterpret_cast<uint4 *>(linc)[i] = incv[r];
+        }
+#pragma unroll
+        for (int r = 0; r < EPT; r++) {
+            const int t = tid + r * BT;
+            if (t < TE) lverts[t] = vert[r];
+        }
+        // the zero slot (index TE): the target of an unpaired second contribution
+        if (tid < S) stage[TE * S + tid] = (T)0;
+        if (tid == 0) lverts[TE] = 0u;
+        __syncthreads();
+
+        // ---- A: scaled gradients + coefficients of every tile element, one slot each
+#pragma unroll
+        for (int r = 0; r < EPT; r++) {
+            const int t = tid + r * BT;
+            if (t >= TE) continue;
+            const int vi[4] = {(int)(vert[r] & 0xFF), (int)((vert[r] >> 8) & 0xFF),
+                               (int)((vert[r] >> 16) & 0xFF), (int)(vert[r] >> 24)};
+            double c[4][3], g[4][3];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) c[i][d] = lc[vi[i] * 3 + d];
+            mf::elem_grad(c, g);
+            T *sl = stage + t * S;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                *reinterpret_cast<T2 *>(sl + 2 * i) = T2{(T)g[i][0], (T)g[i][1]};
+                sl[8 + i] = (T)g[i][2];
+            }
             if constexpr (ELA) {
-                const double2 lm = reinterpret_cast<const double2 *>(coef)[elemId[r]];
-                kc0[r] = lm.x;
-                kc1[r] = lm.y;
+                // src/assembly.cc:386-409 with per-element (lambda, mu)
+                const double cd = kc0[r] + 2.0 * kc1[r], co = kc0[r] + kc1[r], cm = kc1[r];
+                *reinterpret_cast<T2 *>(sl + 12) = T2{(T)cd, (T)co};
+                *reinterpret_cast<T2 *>(sl + 14) = T2{(T)cm, (T)0};
             } else {
-                kc0[r] = coef[elemId[r]];
+                *reinterpret_cast<T2 *>(sl + 12) = T2{(T)kc0[r], (T)0};
             }
         }
-    }
+        // this tile's task words, before the registers take the next tile's
+        uint32_t tinf[PB];
+        int32_t txp[PB];
+        M tm[PB];
+#pragma unroll
+        for (int k = 0; k < PB; k++) {
+            tinf[k] = pinf[k];
+            txp[k] = pxp[k];
+            tm[k] = pm[k];
+        }
+        int dg, m0, mstep;
+        mf::task_blocks(R, h.n_task, NW, wave, dg, m0, mstep);
+        const int nEB = (h.n_task + 63) / 64;
+        __syncthreads();
 
-    // ---- static data into LDS
-#pragma unroll
-    for (int r = 0; r < NPT; r++) {
-        const int i = tid + r * BT;
-        if (i < R) {
-            lri[i] = rinf[r];
-            lrnode[i] = nodeId[r];
+        // ---- the next tile's static data, in flight during B
+        const int nslot = slot + G;
+        const bool has_next = nslot < per && xcd * per + nslot < ntiles;
+        mf::TileHdr hn = h;
+        if (has_next) {
+            hn = tiles[xcd * per + nslot];
+            load_static(hn);
         }
-    }
-#pragma unroll
-    for (int r = 0; r < IPT; r++) {
-        const int i = tid + r * BT;
-        if (i < h.n_inc / 8) reinterpret_cast<uint4 *>(linc)[i] = incv[r];
-    }
-#pragma unroll
-    for (int r = 0; r < EPT; r++) {
-        const int t = tid + r * BT;
-        if (t < TE) lverts[t] = vert[r];
-    }
-#pragma unroll
-    for (int r = 0; r < NPT; r++) {
-        const int i = tid + r * BT;
-        if (i < NN) {
-            lc[i * 3 + 0] = cx[r][0];
-            lc[i * 3 + 1] = cx[r][1];
-            lc[i * 3 + 2] = cx[r][2];
-        }
-    }
-    __syncthreads();
+        bool pending = has_next;  // the next tile's gathers are still to be issued
 
-    // ---- A: scaled gradients + coefficients of every tile element, one slot each
-#pragma unroll
-    for (int r = 0; r < EPT; r++) {
-        const int t = tid + r * BT;
-        if (t >= TE) continue;
-        const int vi[4] = {(int)(vert[r] & 0xFF), (int)((vert[r] >> 8) & 0xFF),
-                           (int)((vert[r] >> 16) & 0xFF), (int)(vert[r] >> 24)};
-        double c[4][3], g[4][3];
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int d = 0; d < 3; d++) c[i][d] = lc[vi[i] * 3 + d];
-        mf::elem_grad(c, g);
-        T *sl = stage + t * S;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            *reinterpret_cast<T2 *>(sl + 2 * i) = T2{(T)g[i][0], (T)g[i][1]};
-            sl[8 + i] = (T)g[i][2];
-        }
-        if constexpr (ELA) {
-            // src/assembly.cc:386-409 with per-element (lambda, mu)
-            const double cd = kc0[r] + 2.0 * kc1[r], co = kc0[r] + kc1[r], cm = kc1[r];
-            *reinterpret_cast<T2 *>(sl + 12) = T2{(T)cd, (T)co};
-            *reinterpret_cast<T2 *>(sl + 14) = T2{(T)cm, (T)0};
-        } else {
-            *reinterpret_cast<T2 *>(sl + 12) = T2{(T)kc0[r], (T)0};
-        }
-    }
-    __syncthreads();
-
-    // ---- B1: diagonal blocks, one lane per row: the whole incidence list, ascending,
-    // four entries per LDS read with their stage reads issued together
-    if (wave < nDB) {
-        const int r = wave * 64 + lane;
-        if (r < R) {
-            const int4 ri = lri[r];
-            const int incOff = ri.z & 0xFFFF, val = (ri.z >> 16) & 0xFFFF;
+        // ---- B1: diagonal blocks, one lane per row: the whole incidence list, ascending,
+        // four entries per LDS read with their stage reads issued together
+        if (dg >= 0) {
+            const int r = dg * 64 + lane;
+            const bool on = r < R;
+            const int4 ri = on ? lri[r] : make_int4(0, 0, 0, 0);
+            const int incOff = ri.z & 0xFFFF, val = on ? (ri.z >> 16) & 0xFFFF : 0;
             const int64_t dpos = (int64_t)ri.x + ((ri.y >> 16) & 0xFF);
             if constexpr (ELA) {
                 // ela_block(g, g): exactly symmetric (g_d*g_e == g_e*g_d in IEEE), so six
@@ -305,15 +331,23 @@ __global__ void __launch_bounds__(BT)
                         s8 += (double)(p00 * cm[u] + p11 * cm[u] + p22 * cd[u]);
                     }
                 }
+                if (pending) {
+                    load_gathers(hn);
+                    pending = false;
+                }
                 double blk[9] = {s0, s1, s2, s1, s4, s5, s2, s5, s8};
-                double *dv = values + dpos * 9;
+                if (on) {
+                    double *dv = values + dpos * 9;
 #pragma unroll
-                for (int q = 0; q < 9; q++) dv[q] = blk[q];
+                    for (int q = 0; q < 9; q++) dv[q] = blk[q];
+                }
                 const int info = mf::ela_invert_block(blk, (ri.y >> 24) & 7, true);
-                double *dp = prec + (int64_t)lrnode[r] * 9;
+                if (on) {
+                    double *dp = prec + (int64_t)lrnode[r] * 9;
 #pragma unroll
-                for (int q = 0; q < 9; q++) dp[q] = blk[q];
-                if (info != 0 && singular) atomicAdd(singular, 1);
+                    for (int q = 0; q < 9; q++) dp[q] = blk[q];
+                    if (info != 0 && singular) atomicAdd(singular, 1);
+                }
             } else {
                 double acc = 0.0;
                 for (int i = 0; i < val; i += 4) {
@@ -334,78 +368,103 @@ __global__ void __launch_bounds__(BT)
                         acc += (double)x;
                     }
                 }
-                values[dpos] = acc;
-                prec[lrnode[r]] = 1.0 / acc;
+                if (pending) {
+                    load_gathers(hn);
+                    pending = false;
+                }
+                if (on) {
+                    values[dpos] = acc;
+                    prec[lrnode[r]] = 1.0 / acc;
+                }
             }
         }
-    }
 
-    // ---- B2: off-diagonal entries, one lane per CSR entry: the elements of the row's
-    // list that hold the column, two per iteration (independent LDS chains, summed in
-    // order)
+        // ---- B2: off-diagonal tasks, one lane per block pair: the elements of the row's
+        // list that hold the column, two per iteration (independent LDS chains, summed in
+        // order); the block goes to row a, its transpose to row c
 #pragma clang loop unroll(disable)
-    for (int k = 0; k < PB; k++) {
-        const int m = m0 + k * mstep;
-        if (m0 < 0 || m >= nEB) break;
-        const uint32_t inf = pinf[0];
-        M msk = pm[0];
+        for (int k = 0; k < PB; k++) {
+            const int m = m0 + k * mstep;
+            if (m0 < 0 || m >= nEB) break;
+            const uint32_t inf = tinf[0];
+            const int32_t xp = txp[0];
+            M msk = tm[0];
 #pragma unroll
-        for (int s = 0; s + 1 < PB; s++) {
-            pinf[s] = pinf[s + 1];
-            pm[s] = pm[s + 1];
-        }
-        const int ra = inf & 0xFF, cl = (inf >> 8) & 0xFF;
-        if (inf == kNoEntry || cl == ra) continue;  // past the end, or the diagonal (B1)
-        const int4 ri = lri[ra];
-        const int incOff = ri.z & 0xFFFF;
-        const int oa = m * 64 + lane - ri.w;
-        double acc[NA];
+            for (int q = 0; q + 1 < PB; q++) {
+                tinf[q] = tinf[q + 1];
+                txp[q] = txp[q + 1];
+                tm[q] = tm[q + 1];
+            }
+            const bool on = inf != kNoEntry;  // lanes past the tile's last task idle
+            if (!on) msk = 0;
+            const int ra = inf & 0xFF, cl = (inf >> 8) & 0xFF, oa = inf >> 16;
+            const int4 ri = on ? lri[ra] : make_int4(0, 0, 0, 0);
+            const int incOff = ri.z & 0xFFFF;
+            double acc[NA];
 #pragma unroll
-        for (int q = 0; q < NA; q++) acc[q] = 0.0;
-        while (msk) {
-            const int b0 = lowest_bit(msk);
-            msk &= msk - 1;
-            const bool two = msk != 0;
-            const int b1 = two ? lowest_bit(msk) : b0;
-            if (two) msk &= msk - 1;
-            const int e0 = linc[incOff + b0], e1 = linc[incOff + b1];
-            const int t[2] = {e0 >> 2, e1 >> 2}, a[2] = {e0 & 3, e1 & 3};
-            T ga[2][3], gk[2][3], c0[2], c1[2], c2[2];
+            for (int q = 0; q < NA; q++) acc[q] = 0.0;
+            while (msk) {
+                // two contributions per iteration; a missing second one reads the
+                // all-zero slot TE, whose products are +0.0 and leave the sums unchanged
+                const int b0 = lowest_bit(msk);
+                msk &= msk - 1;
+                const bool two = msk != 0;
+                const int b1 = two ? lowest_bit(msk) : b0;
+                if (two) msk &= msk - 1;
+                const int e0 = linc[incOff + b0], e1 = linc[incOff + b1];
+                const int t[2] = {e0 >> 2, two ? e1 >> 2 : TE}, a[2] = {e0 & 3, e1 & 3};
+                T ga[2][3], gk[2][3], c0[2], c1[2], c2[2];
 #pragma unroll
-            for (int u = 0; u < 2; u++) {
-                const uint32_t vv = lverts[t[u]];
-                const int kk = (int)(vv & 0xFF) == cl ? 0
-                               : (int)((vv >> 8) & 0xFF) == cl ? 1
-                               : (int)((vv >> 16) & 0xFF) == cl ? 2 : 3;
-                const T *sl = stage + t[u] * S;
-                stage_grad<T, T2>(sl, a[u], ga[u]);
-                stage_grad<T, T2>(sl, kk, gk[u]);
-                if constexpr (ELA) {
-                    const T2 c01 = *reinterpret_cast<const T2 *>(sl + 12);
-                    c0[u] = c01.x;
-                    c1[u] = c01.y;
-                    c2[u] = sl[14];
-                } else {
-                    c0[u] = sl[12];
+                for (int u = 0; u < 2; u++) {
+                    const uint32_t vv = lverts[t[u]];
+                    const int kk = (int)(vv & 0xFF) == cl ? 0
+                                   : (int)((vv >> 8) & 0xFF) == cl ? 1
+                                   : (int)((vv >> 16) & 0xFF) == cl ? 2 : 3;
+                    const T *sl = stage + t[u] * S;
+                    stage_grad<T, T2>(sl, a[u], ga[u]);
+                    stage_grad<T, T2>(sl, kk, gk[u]);
+                    if constexpr (ELA) {
+                        const T2 c01 = *reinterpret_cast<const T2 *>(sl + 12);
+                        c0[u] = c01.x;
+                        c1[u] = c01.y;
+                        c2[u] = sl[14];
+                    } else {
+                        c0[u] = sl[12];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    if constexpr (ELA) {
+                        T bl[9];
+                        mf::ela_block(ga[u], gk[u], c0[u], c1[u], c2[u], bl);
+#pragma unroll
+                        for (int q = 0; q < 9; q++) acc[q] += (double)bl[q];
+                    } else {
+                        const T x = c0[u] * mf::lap_entry(ga[u], gk[u]);
+                        acc[0] += (double)x;
+                    }
                 }
             }
+            if (pending) {
+                load_gathers(hn);
+                pending = false;
+            }
+            if (on) {
+                double *dst = values + ((int64_t)ri.x + oa) * NA;
 #pragma unroll
-            for (int u = 0; u < 2; u++) {
-                if (u == 1 && !two) break;
-                if constexpr (ELA) {
-                    T bl[9];
-                    mf::ela_block(ga[u], gk[u], c0[u], c1[u], c2[u], bl);
+                for (int q = 0; q < NA; q++) dst[q] = acc[q];
+                if (xp >= 0) {
+                    double *dx = values + (int64_t)xp * NA;
 #pragma unroll
-                    for (int q = 0; q < 9; q++) acc[q] += (double)bl[q];
-                } else {
-                    const T x = c0[u] * mf::lap_entry(ga[u], gk[u]);
-                    acc[0] += (double)x;
+                    for (int q = 0; q < NA; q++) dx[q] = acc[(q % 3) * 3 + q / 3];  // transpose
                 }
             }
         }
-        double *dst = values + ((int64_t)ri.x + oa) * NA;
-#pragma unroll
-        for (int q = 0; q < NA; q++) dst[q] = acc[q];
+        if (pending) load_gathers(hn);
+        __syncthreads();  // every wave is done with this tile's LDS
+        if (!has_next) break;
+        h = hn;
+        slot = nslot;
     }
 }
 
@@ -455,8 +514,9 @@ int upload_as(D **dst, const std::vector<S> &src)
 void free_plan(mf_plan *p)
 {
     if (!p) return;
-    void *ptrs[] = {p->tiles, p->tile_elems, p->tile_verts, p->tile_nodes, p->tile_rowinfo,
-                    p->inc, p->ent_mask, p->ent_info, p->cb, p->isolated};
+    void *ptrs[] = {p->tiles,     p->tile_elems, p->tile_verts, p->tile_nodes,
+                    p->tile_rowinfo, p->inc,     p->task_mask,  p->task_info,
+                    p->task_xpos, p->cb,         p->isolated};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     delete p;
@@ -471,12 +531,21 @@ int launch_tiled(mf_plan *p, const double *coord, const double *coef, double *va
     if (hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                             hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
         return MF_ERR_HIP;
-    const int per = (p->ntiles + kXcd - 1) / kXcd;
     if (p->ntiles > 0) {
-        hipLaunchKernelGGL(kern, dim3(per * kXcd), dim3(BT), lds, s, p->tiles, p->ntiles,
+        // persistent grid: as many workgroups per XCD as fit at once, at most one per
+        // slot of its tile range
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(kern),
+                                                         BT, lds) != hipSuccess)
+            return MF_ERR_HIP;
+        const int per = (p->ntiles + kXcd - 1) / kXcd;
+        const int G = std::max(1, std::min(per, std::max(1, per_cu) * std::max(1, cus / kXcd)));
+        hipLaunchKernelGGL(kern, dim3(G * kXcd), dim3(BT), lds, s, p->tiles, p->ntiles, G,
                            p->tile_elems, p->tile_verts, p->tile_nodes, p->tile_rowinfo,
-                           reinterpret_cast<const uint4 *>(p->inc), p->ent_mask, p->ent_info,
-                           coord, coef, values, prec, singular);
+                           reinterpret_cast<const uint4 *>(p->inc), p->task_mask, p->task_info,
+                           p->task_xpos, coord, coef, values, prec, singular);
         MF_CHECK_LAUNCH();
     }
     if (p->n_isolated > 0) {
@@ -493,9 +562,9 @@ int launch_variant(mf_plan *p, const double *coord, const double *coef, double *
 {
     const bool wide = p->mask_words == 2;
     switch (p->block) {
-    case 256:
-        return wide ? launch_tiled<OP, PREC, 256, 2>(p, coord, coef, values, prec, singular, s)
-                    : launch_tiled<OP, PREC, 256, 1>(p, coord, coef, values, prec, singular, s);
+    case 1024:
+        return wide ? launch_tiled<OP, PREC, 1024, 2>(p, coord, coef, values, prec, singular, s)
+                    : launch_tiled<OP, PREC, 1024, 1>(p, coord, coef, values, prec, singular, s);
     case 512:
         return wide ? launch_tiled<OP, PREC, 512, 2>(p, coord, coef, values, prec, singular, s)
                     : launch_tiled<OP, PREC, 512, 1>(p, coord, coef, values, prec, singular, s);
@@ -504,23 +573,25 @@ int launch_variant(mf_plan *p, const double *coord, const double *coef, double *
 }
 
 struct Resolved {
-    int budget, block, max_rows, slot64, slot32;
+    int budget, block, max_rows, slot64, slot32, pair_window;
 };
 
 int resolve_params(int op, const mf_plan_params *params, Resolved *r)
 {
     const int S = op == MF_OP_ELA ? mf::kStageEla : mf::kStageLap;
     r->budget = 64 * 1024;
-    r->block = 256;
+    r->block = 512;
     r->max_rows = op == MF_OP_ELA ? 64 : 128;
+    r->pair_window = kDefaultPairWindow;
     if (params) {
+        if (params->pair_window != 0) r->pair_window = std::max(params->pair_window, 0);
         if (params->tile_lds_bytes > 0) r->budget = params->tile_lds_bytes;
         if (params->block_threads > 0) r->block = params->block_threads;
         if (params->max_tile_rows > 0) r->max_rows = params->max_tile_rows;
     }
     r->slot64 = S * 8;
     r->slot32 = S * 4;
-    if (r->block != 256 && r->block != 512) return MF_ERR_ARG;
+    if (r->block != 512 && r->block != 1024) return MF_ERR_ARG;
     if (r->budget < 4096 || r->budget > mf::kMaxLdsBytes || r->max_rows > mf::kMaxTileRows)
         return MF_ERR_ARG;
     return MF_OK;
@@ -531,7 +602,7 @@ int build_checked(mf::PlanHost &h, int op, int nE, int N, int nnz, const int *e2
                   const Resolved &r, int *lds64, int *lds32)
 {
     int st = mf::build_plan(h, op, nE, N, nnz, e2n, row, col, cb, coord, r.budget, r.max_rows,
-                            r.slot64, r.block);
+                            r.slot64, r.block, r.pair_window);
     if (st != MF_OK) return st;
     st = mf::validate_plan(h, row, col, e2n, r.slot64);
     if (st != MF_OK) return st;
@@ -546,18 +617,19 @@ int build_checked(mf::PlanHost &h, int op, int nE, int N, int nnz, const int *e2
     return MF_OK;
 }
 
-// stats: tiles, staged tile elements, CSR entries, incidence entries, plan bytes, LDS
-// bytes per workgroup (FP64), block threads, largest tile (rows)
+// stats: tiles, staged tile elements, off-diagonal tasks, incidence entries, plan
+// bytes, LDS bytes per workgroup (FP64), block threads, largest tile (rows)
 void fill_stats(const mf::PlanHost &h, int lds64, int block, int64_t out[8])
 {
     out[0] = (int64_t)h.tiles.size();
     out[1] = (int64_t)h.tile_elems.size();
-    out[2] = (int64_t)h.ent_info.size();
+    out[2] = (int64_t)h.task_info.size();
     out[3] = (int64_t)h.inc.size();
     out[4] = (int64_t)h.tiles.size() * sizeof(mf::TileHdr) + (int64_t)h.tile_elems.size() * 4 +
              (int64_t)h.tile_verts.size() + (int64_t)h.tile_nodes.size() * 4 +
              (int64_t)h.tile_rowinfo.size() * sizeof(mf::RowInfo) + (int64_t)h.inc.size() * 2 +
-             (int64_t)h.ent_mask.size() * 4 + (int64_t)h.ent_info.size() * 2;
+             (int64_t)h.task_mask.size() * 4 + (int64_t)h.task_info.size() * 4 +
+             (int64_t)h.task_xpos.size() * 4;
     out[5] = lds64;
     out[6] = block;
     out[7] = h.max_rows;
@@ -607,8 +679,9 @@ extern "C" int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nb
     if (rc == MF_OK) rc = upload(&p->tile_nodes, h.tile_nodes);
     if (rc == MF_OK) rc = upload_as(&p->tile_rowinfo, h.tile_rowinfo);
     if (rc == MF_OK) rc = upload(&p->inc, h.inc);
-    if (rc == MF_OK) rc = upload(&p->ent_mask, h.ent_mask);
-    if (rc == MF_OK) rc = upload(&p->ent_info, h.ent_info);
+    if (rc == MF_OK) rc = upload(&p->task_mask, h.task_mask);
+    if (rc == MF_OK) rc = upload(&p->task_info, h.task_info);
+    if (rc == MF_OK) rc = upload(&p->task_xpos, h.task_xpos);
     if (rc == MF_OK) rc = upload(&p->isolated, h.isolated);
     if (rc == MF_OK && checkBounds && operatorID == MF_OP_ELA)
         rc = upload(&p->cb, std::vector<int32_t>(checkBounds, checkBounds + (size_t)nbNodes * 3));

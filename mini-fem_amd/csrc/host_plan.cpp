@@ -16,7 +16,7 @@ namespace {
 struct Partitioner {
     int N, nE, max_rows, budget, slot, block;
     bool ela;
-    const int *e2n, *row;
+    const int *e2n, *row, *col;
     const std::vector<int> &n2eI, &n2eV;
     const double *coord;
     std::vector<int> order;
@@ -25,10 +25,10 @@ struct Partitioner {
     int stampId = 0;
 
     Partitioner(int N_, int nE_, int mr, int bud, int slot_, int block_, bool ela_,
-                const int *e2n_, const int *row_, const std::vector<int> &I,
+                const int *e2n_, const int *row_, const int *col_, const std::vector<int> &I,
                 const std::vector<int> &V, const double *c, std::vector<int> rows)
         : N(N_), nE(nE_), max_rows(mr), budget(bud), slot(slot_), block(block_), ela(ela_),
-          e2n(e2n_), row(row_), n2eI(I), n2eV(V), coord(c), order(std::move(rows)),
+          e2n(e2n_), row(row_), col(col_), n2eI(I), n2eV(V), coord(c), order(std::move(rows)),
           elemStamp(nE_, -1), nodeStamp(N_, -1)
     {
     }
@@ -42,9 +42,16 @@ struct Partitioner {
         stampId++;
         for (int i = lo; i < hi; i++) nodeStamp[order[i]] = stampId;
         int64_t TE = 0, NN = R, inc = 0, ent = 0;
+        // tasks: at most one per off-diagonal block, a pair of the tile's rows once
         for (int i = lo; i < hi; i++) {
             const int r = order[i];
-            ent += row[r + 1] - row[r];
+            for (int q = row[r]; q < row[r + 1]; q++) {
+                const int c = col[q] - 1;
+                if (c != r && (nodeStamp[c] != stampId || c > r)) ent++;
+            }
+        }
+        for (int i = lo; i < hi; i++) {
+            const int r = order[i];
             inc += (n2eI[r + 1] - n2eI[r] + kIncAlign - 1) / kIncAlign * kIncAlign;
             for (int j = n2eI[r]; j < n2eI[r + 1]; j++) {
                 const int e = n2eV[j];
@@ -132,7 +139,7 @@ struct Partitioner {
 
 int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, const int *row,
                const int *col, const int *checkBounds, const double *coord, int lds_budget,
-               int max_rows, int slot_bytes, int block)
+               int max_rows, int slot_bytes, int block, int pair_window)
 {
     p = PlanHost{};
     p.op = op;
@@ -173,15 +180,28 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
     for (int i = 0; i < N; i++)
         if ((n2eI[i + 1] > n2eI[i]) != (row[i + 1] > row[i])) return MF_ERR_ARG;
 
-    Partitioner part(N, nE, max_rows, lds_budget, slot_bytes, block, ela, e2n, row, n2eI, n2eV,
-                     coord, rowsAll);
+    Partitioner part(N, nE, max_rows, lds_budget, slot_bytes, block, ela, e2n, row, col, n2eI,
+                     n2eV, coord, rowsAll);
     if (coord) part.rcb(0, (int)rowsAll.size());
     else part.bfs_chunks();
+
+    // tile of every row node, and the kernel's dispatch order: tile i runs on XCD i / per
+    // as that XCD's (i % per)-th workgroup
+    const int T = (int)part.leaves.size(), per = (T + kXcds - 1) / kXcds;
+    std::vector<int> nodeTile((size_t)N, -1);
+    for (int i = 0; i < T; i++)
+        for (int q = part.leaves[i].first; q < part.leaves[i].second; q++) nodeTile[part.order[q]] = i;
+    auto find_pos = [&](int c, int a) -> int64_t {  // CSR position of block (c, a)
+        for (int q = row[c]; q < row[c + 1]; q++)
+            if (col[q] == a + 1) return q;
+        return -1;
+    };
 
     std::vector<int> nodeLocal((size_t)N, -1), elemLocal((size_t)nE, -1);
     std::vector<int> rows, tel;
     const int MW = p.mask_words;
-    for (const auto &lf : part.leaves) {
+    for (int ti = 0; ti < T; ti++) {
+        const auto &lf = part.leaves[ti];
         rows.assign(part.order.begin() + lf.first, part.order.begin() + lf.second);
         std::sort(rows.begin(), rows.end());
         const int R = (int)rows.size();
@@ -224,8 +244,8 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
 
         // incidence lists (ascending element = ascending tile element) and entries
         h.inc_start = (int)p.inc.size();
-        h.ent_start = (int)p.ent_info.size();
-        int incOff = 0, entOff = 0;
+        h.task_start = (int)p.task_info.size();
+        int incOff = 0;
         for (int ra = 0; ra < R; ra++) {
             const int a = rows[ra], len = row[a + 1] - row[a];
             const int val = n2eI[a + 1] - n2eI[a];
@@ -237,8 +257,7 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
             if (ela && checkBounds)
                 for (int k = 0; k < 3; k++)
                     if (checkBounds[(int64_t)k * N + a] != 0) mask |= 1 << k;
-            p.tile_rowinfo.push_back(
-                RowInfo{row[a], len | doff << 16 | mask << 24, incOff | val << 16, entOff});
+            p.tile_rowinfo.push_back(RowInfo{row[a], len | doff << 16 | mask << 24, incOff | val << 16, 0});
             for (int j = n2eI[a]; j < n2eI[a + 1]; j++) {
                 const int e = n2eV[j];
                 int va = 0;
@@ -248,31 +267,46 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
             }
             for (int o = 0; o < len; o++) {
                 const int c = col[row[a] + o] - 1;
-                if (c < 0 || c >= N || nodeLocal[c] < 0) return MF_ERR_ARG;
+                if (c < 0 || c >= N || nodeLocal[c] < 0 || o > 0xFF) return MF_ERR_ARG;
+                if (c == a) continue;
+                // pair (a,c)/(c,a): computed once, by the row with the smaller node id in
+                // one tile, by the earlier tile across tiles
+                const int tj = nodeTile[c];
+                int64_t xpos = -1;
+                if (tj == ti) {
+                    if (c < a) continue;
+                    xpos = find_pos(c, a);
+                } else if (pair_window > 0 && tj / per == ti / per && std::abs(tj - ti) <= pair_window) {
+                    if (tj < ti) continue;
+                    xpos = find_pos(c, a);
+                }
+                if (tj != ti && xpos < 0 && pair_window > 0 && tj / per == ti / per &&
+                    std::abs(tj - ti) <= pair_window)
+                    return MF_ERR_ARG;  // the pattern is not symmetric
+                if (tj == ti && xpos < 0) return MF_ERR_ARG;
                 uint64_t m = 0;
-                if (c != a)
-                    for (int j = n2eI[a]; j < n2eI[a + 1]; j++) {
-                        const int *v = e2n + (int64_t)n2eV[j] * 4;
-                        if (v[0] - 1 == c || v[1] - 1 == c || v[2] - 1 == c || v[3] - 1 == c)
-                            m |= 1ull << (j - n2eI[a]);
-                    }
-                p.ent_info.push_back((uint16_t)(ra | nodeLocal[c] << 8));
-                for (int w = 0; w < MW; w++) p.ent_mask.push_back((uint32_t)(m >> (32 * w)));
+                for (int j = n2eI[a]; j < n2eI[a + 1]; j++) {
+                    const int *v = e2n + (int64_t)n2eV[j] * 4;
+                    if (v[0] - 1 == c || v[1] - 1 == c || v[2] - 1 == c || v[3] - 1 == c)
+                        m |= 1ull << (j - n2eI[a]);
+                }
+                p.task_info.push_back((uint32_t)(ra | nodeLocal[c] << 8 | o << 16));
+                for (int w = 0; w < MW; w++) p.task_mask.push_back((uint32_t)(m >> (32 * w)));
+                p.task_xpos.push_back((int32_t)xpos);
             }
             while (p.inc.size() % kIncAlign) p.inc.push_back(0);
             incOff = (int)p.inc.size() - h.inc_start;
-            entOff += len;
         }
         while (p.inc.size() % 8) p.inc.push_back(0);
         h.n_inc = (int)p.inc.size() - h.inc_start;
-        h.n_ent = entOff;
+        h.n_task = (int)p.task_info.size() - h.task_start;
         p.tiles.push_back(h);
         p.max_node = std::max(p.max_node, h.n_node);
         p.max_elem = std::max(p.max_elem, TE);
         p.max_rows = std::max(p.max_rows, R);
         for (int i = h.node_start; i < (int)p.tile_nodes.size(); i++) nodeLocal[p.tile_nodes[i]] = -1;
         for (int e : tel) elemLocal[e] = -1;
-        if (p.inc.size() > (size_t)INT32_MAX - 100000 || p.ent_info.size() > (size_t)INT32_MAX / 2)
+        if (p.inc.size() > (size_t)INT32_MAX - 100000 || p.task_info.size() > (size_t)INT32_MAX / 2)
             return MF_ERR_NOMEM;
     }
     return MF_OK;
@@ -285,7 +319,8 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
     const int MW = p.mask_words;
     if (MW < 1 || MW > 2) return MF_ERR_STATE;
     if (p.tile_verts.size() != p.tile_elems.size() * 4) return MF_ERR_STATE;
-    if (p.ent_mask.size() != p.ent_info.size() * MW) return MF_ERR_STATE;
+    if (p.task_mask.size() != p.task_info.size() * MW || p.task_xpos.size() != p.task_info.size())
+        return MF_ERR_STATE;
     std::vector<uint8_t> hit((size_t)p.nnz, 0), rowSeen((size_t)p.N, 0);
     for (int i : p.isolated) {
         if (i < 0 || i >= p.N || row[i + 1] != row[i] || rowSeen[i]) return MF_ERR_STATE;
@@ -307,10 +342,25 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
         if (h.inc_start < 0 || h.inc_start % 8 || h.n_inc < 0 || h.n_inc % 8 ||
             h.n_inc > kMaxTileInc || (size_t)h.inc_start + h.n_inc > p.inc.size())
             return MF_ERR_STATE;
-        if (h.ent_start < 0 || h.n_ent < 0 || (size_t)h.ent_start + h.n_ent > p.ent_info.size())
+        if (h.task_start < 0 || h.n_task < 0 || (size_t)h.task_start + h.n_task > p.task_info.size())
             return MF_ERR_STATE;
-        const int blocks = diag_blocks(ela, R) + (h.n_ent + kLanes - 1) / kLanes;
-        if (blocks > kMaxTileBlocks || (blocks + NW - 1) / NW > kMaxTileBlocks / NW) return MF_ERR_STATE;
+        const int nDB = diag_blocks(ela, R), nEB = (h.n_task + kLanes - 1) / kLanes;
+        if (nDB + nEB > kMaxTileBlocks || nDB > NW) return MF_ERR_STATE;
+        {  // every block lands in one per-wave slot of the kernel, once
+            std::vector<int> seen(nEB, 0), dseen(nDB, 0);
+            for (int w = 0; w < NW; w++) {
+                int dg, m0, mstep;
+                task_blocks(R, h.n_task, NW, w, dg, m0, mstep);
+                if (dg >= nDB) return MF_ERR_STATE;
+                if (dg >= 0) dseen[dg]++;
+                for (int k = 0; k < kMaxTileBlocks / NW && m0 >= 0; k++)
+                    if (m0 + k * mstep < nEB) seen[m0 + k * mstep]++;
+            }
+            for (int m = 0; m < nEB; m++)
+                if (seen[m] != 1) return MF_ERR_STATE;
+            for (int i = 0; i < nDB; i++)
+                if (dseen[i] != 1) return MF_ERR_STATE;
+        }
         if (tile_lds_layout(h.n_node, h.n_elem, R, h.n_inc, slot_bytes).total + 16 > kMaxLdsBytes)
             return MF_ERR_STATE;
         const int32_t *nodes = p.tile_nodes.data() + h.node_start;
@@ -333,9 +383,12 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
                 if (verts[t * 4 + k] >= h.n_node || nodes[verts[t * 4 + k]] != e2n[(int64_t)e * 4 + k] - 1)
                     return MF_ERR_STATE;
         }
+        for (int i = h.inc_start; i < h.inc_start + h.n_inc; i++)
+            if ((p.inc[i] >> 2) >= std::max(h.n_elem, 1)) return MF_ERR_STATE;  // padding too
         std::fill(cnt.begin(), cnt.end(), 0);
         for (int q = 0; q < h.n_elem * 4; q++) cnt[verts[q]]++;
-        int incNext = 0, entNext = 0;
+        int incNext = 0, tk = h.task_start;
+        const int tkEnd = h.task_start + h.n_task;
         for (int ra = 0; ra < R; ra++) {
             const int a = nodes[ra];
             const RowInfo ri = p.tile_rowinfo[h.rinfo_start + ra];
@@ -345,11 +398,11 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
             if (ri.rowptr != row[a] || len != row[a + 1] - row[a] || len < 1 || doff >= len ||
                 col[row[a] + doff] != a + 1 || mask > 7 || (!ela && mask))
                 return MF_ERR_STATE;
-            if (incOff != incNext || incOff % kIncAlign || val < 1 || val > 32 * MW || incOff + val > h.n_inc ||
-                ri.ent != entNext || ri.ent + len > h.n_ent)
+            if (incOff != incNext || incOff % kIncAlign || val < 1 || val > 32 * MW ||
+                incOff + val > h.n_inc)
                 return MF_ERR_STATE;
             incNext += (val + kIncAlign - 1) / kIncAlign * kIncAlign;
-            entNext += len;
+            if (hit[(size_t)row[a] + doff]++) return MF_ERR_STATE;  // the diagonal block
             // the incidence list: every element of the tile holding a, ascending
             int last = -1;
             for (int i = 0; i < val; i++) {
@@ -360,25 +413,31 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
                 for (int k = 0; k < 4; k++) holds[verts[t * 4 + k]] |= 1ull << i;
             }
             if (cnt[ra] != val) return MF_ERR_STATE;
-            // the row's entries: one per CSR position, each mask exact
-            for (int o = 0; o < len; o++) {
-                const size_t e = (size_t)h.ent_start + ri.ent + o;
-                const int era = p.ent_info[e] & 0xFF, ec = p.ent_info[e] >> 8;
-                if (era != ra || ec >= h.n_node || nodes[ec] != col[row[a] + o] - 1) return MF_ERR_STATE;
-                if (hit[(size_t)row[a] + o]++) return MF_ERR_STATE;
-                uint64_t m = p.ent_mask[e * MW];
-                if (MW == 2) m |= (uint64_t)p.ent_mask[e * MW + 1] << 32;
-                const uint64_t want = ec == ra ? 0 : holds[ec];
-                if (m != want || (ec != ra && m == 0)) return MF_ERR_STATE;
+            // the row's tasks: exact masks, each block and its transpose written once
+            for (; tk < tkEnd && (int)(p.task_info[tk] & 0xFF) == ra; tk++) {
+                const uint32_t inf = p.task_info[tk];
+                const int ec = (inf >> 8) & 0xFF, oa = inf >> 16;
+                if (ec >= h.n_node || ec == ra || oa >= len) return MF_ERR_STATE;
+                const int c = nodes[ec];
+                if (col[row[a] + oa] != c + 1) return MF_ERR_STATE;
+                if (hit[(size_t)row[a] + oa]++) return MF_ERR_STATE;
+                uint64_t m = p.task_mask[(size_t)tk * MW];
+                if (MW == 2) m |= (uint64_t)p.task_mask[(size_t)tk * MW + 1] << 32;
+                if (m != holds[ec] || m == 0) return MF_ERR_STATE;
+                const int64_t xp = p.task_xpos[tk];
+                if (xp >= 0) {
+                    if (xp < row[c] || xp >= row[c + 1] || col[xp] != a + 1 || hit[xp]++)
+                        return MF_ERR_STATE;
+                } else if (xp != -1) {
+                    return MF_ERR_STATE;
+                }
             }
             for (int i = 0; i < val; i++) {
                 const int t = p.inc[(size_t)h.inc_start + incOff + i] >> 2;
                 for (int k = 0; k < 4; k++) holds[verts[t * 4 + k]] = 0;
             }
         }
-        if (entNext != h.n_ent || incNext > h.n_inc || h.n_inc - incNext >= 8) return MF_ERR_STATE;
-        for (int i = h.inc_start; i < h.inc_start + h.n_inc; i++)
-            if ((p.inc[i] >> 2) >= std::max(h.n_elem, 1)) return MF_ERR_STATE;  // padding too
+        if (tk != tkEnd || incNext > h.n_inc || h.n_inc - incNext >= 8) return MF_ERR_STATE;
     }
     for (int i = 0; i < p.N; i++)
         if (!rowSeen[i]) return MF_ERR_STATE;

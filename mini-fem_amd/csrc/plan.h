@@ -11,6 +11,10 @@
 //      - a row's incident elements are kept in LDS as an ascending "incidence list";
 //      - an off-diagonal block (row a, column c) is one lane, which walks the bits of a
 //        precomputed mask over row a's incidence list (the elements that also hold c);
+//        block (c,a) is its exact transpose (same elements in the same order, and
+//        IEEE products commute), so a pair is computed once and written twice when
+//        c is a row of the same tile, or of a tile that runs close in time on the same
+//        XCD (the partial cache lines then merge in its L2);
 //      - a diagonal block (the whole incidence list) is one lane, which also inverts
 //        it into the block-Jacobi prec (prec_init + prec_inversion fused).
 // Every CSR entry and prec block is written exactly once; values need no zero-fill.
@@ -29,14 +33,15 @@
 namespace mf {
 
 constexpr int kLanes = 64;               // lanes per wave = CSR blocks per work block
-constexpr int kMaxTileElems = 2048;      // incidence entries carry an element index < 2^14
+constexpr int kMaxTileElems = 1024;      // staged elements per tile (the zero slot is extra)
 constexpr int kMaxTileNodes = 256;       // tile-local node ids are 8-bit
 constexpr int kMaxTileRows = 256;        // tile-local row ids are 8-bit
-constexpr int kMaxTileInc = 8192;        // incidence entries per tile (16-bit offsets)
-constexpr int kMaxTileBlocks = 32;       // work blocks (diagonal + entry) per tile
+constexpr int kMaxTileInc = 4096;        // incidence entries per tile
+constexpr int kMaxTileBlocks = 16;       // work blocks (diagonal + task) per tile
 constexpr int kMaxValence = 64;          // incident elements per row (mask width)
 constexpr int kMaxLdsBytes = 160 * 1024; // LDS of one gfx950 CU
 constexpr int kIncAlign = 4;             // row incidence lists start at multiples of 4
+constexpr int kXcds = 8;                 // MI355X: workgroups are dealt to 8 XCDs round-robin
 
 // Element-stage slot of one tile element, in units of the stage type (double or float):
 // the (x,y) gradient pairs of vertices 0..3 at 2i (16-byte aligned for double2 reads),
@@ -50,9 +55,10 @@ constexpr int kStageLap = 14;
 
 MF_HOST_DEVICE inline int align16(int x) { return (x + 15) & ~15; }
 
-// LDS of one tile: [element stage: n_elem slots] [row info: n_rows x 16 B] [node
-// coordinates: n_node x 24 B] [row node ids: n_rows x 4 B] [element vertices: n_elem x
-// 4 B] [incidence lists: n_inc x 2 B, n_inc a multiple of 8].
+// LDS of one tile: [element stage: n_elem + 1 slots, the last one all zero] [row info:
+// n_rows x 16 B] [node coordinates: n_node x 24 B] [row node ids: n_rows x 4 B]
+// [element vertices: n_elem + 1 x 4 B] [incidence lists: n_inc x 2 B, n_inc a multiple
+// of 8].
 struct TileLds {
     int stage, rinfo, coords, rnode, verts, inc, total;
 };
@@ -62,11 +68,11 @@ MF_HOST_DEVICE inline TileLds tile_lds_layout(int n_node, int n_elem, int n_rows
 {
     TileLds L;
     L.stage = 0;
-    L.rinfo = align16(n_elem * slot_bytes);
+    L.rinfo = align16((n_elem + 1) * slot_bytes);
     L.coords = L.rinfo + n_rows * 16;
     L.rnode = L.coords + n_node * 24;
     L.verts = L.rnode + n_rows * 4;
-    L.inc = align16(L.verts + n_elem * 4);
+    L.inc = align16(L.verts + (n_elem + 1) * 4);
     L.total = L.inc + align16(n_inc * 2);
     return L;
 }
@@ -74,13 +80,27 @@ MF_HOST_DEVICE inline TileLds tile_lds_layout(int n_node, int n_elem, int n_rows
 // Diagonal work blocks of a tile: one row per lane.
 MF_HOST_DEVICE inline int diag_blocks(bool, int n_rows) { return (n_rows + kLanes - 1) / kLanes; }
 
+// Work blocks of a tile on `nw` waves: diagonal block i (rows 64i..) on wave i; task block
+// m (tasks 64m..) on the waves without a diagonal block while each holds at most
+// kMaxTileBlocks / nw of them, else on all waves round-robin. A wave's k-th task block is
+// m0 + k * mstep (none when m0 < 0); it holds diagonal block `diag` (-1: none).
+MF_HOST_DEVICE inline void task_blocks(int n_rows, int n_task, int nw, int wave, int &diag,
+                                       int &m0, int &mstep)
+{
+    const int nDB = diag_blocks(true, n_rows), nEB = (n_task + kLanes - 1) / kLanes;
+    const bool spread = nEB > (kMaxTileBlocks / nw) * (nw - nDB);
+    diag = wave < nDB ? wave : -1;
+    m0 = spread ? wave : wave - nDB;
+    mstep = spread ? nw : nw - nDB;
+}
+
 struct TileHdr {
     int32_t elem_start, n_elem;  // into tile_elems / tile_verts (ascending element ids)
     int32_t node_start, n_node;  // into tile_nodes; the first n_rows are the tile's rows
     int32_t n_rows;
     int32_t rinfo_start;         // into tile_rowinfo: n_rows entries
     int32_t inc_start, n_inc;    // into inc (multiples of 8)
-    int32_t ent_start, n_ent;    // into ent_mask / ent_info: the tile rows' CSR entries
+    int32_t task_start, n_task;  // into task_mask / task_info / task_xpos
     int32_t pad[2];
 };
 
@@ -100,10 +120,13 @@ struct RowInfo {
 // Incidence entry: t << 2 | a, t the tile element, a the row node's vertex slot in it.
 // Each row's list starts at a multiple of kIncAlign entries (zero padding between lists:
 // element 0, read but never summed), so a lane reads 4 entries with one 8-byte access.
-// Entries (CSR blocks) of a tile are stored in tile-row order, CSR order inside a row:
-//   ent_info[e] = ra | c << 8  (tile row, tile-local column node; c == ra on the diagonal)
-//   ent_mask[e * mask_words + w]: bit b set when the b-th element of row ra's
-//   incidence list holds node c (zero on the diagonal).
+// Off-diagonal tasks of a tile, in tile-row order and CSR order inside a row:
+//   task_info[e] = ra | c << 8 | oa << 16  (tile row, tile-local column node, offset of
+//                  the block in row ra);
+//   task_mask[e * mask_words + w]: bit b set when the b-th element of row ra's
+//                  incidence list holds node c;
+//   task_xpos[e]:  CSR position of block (c,a), written with the transpose, or -1 when
+//                  c's own tile computes it.
 struct PlanHost {
     int op = 0, nE = 0, N = 0, nnz = 0, block = 256, mask_words = 1;
     std::vector<TileHdr> tiles;
@@ -112,8 +135,9 @@ struct PlanHost {
     std::vector<int32_t> tile_nodes;   // global node ids (0-based)
     std::vector<RowInfo> tile_rowinfo; // per tile row
     std::vector<uint16_t> inc;
-    std::vector<uint32_t> ent_mask;
-    std::vector<uint16_t> ent_info;
+    std::vector<uint32_t> task_mask;
+    std::vector<uint32_t> task_info;
+    std::vector<int32_t> task_xpos;
     std::vector<int32_t> isolated;     // nodes with an empty row (no element)
     int max_node = 0, max_elem = 0, max_rows = 0;
 };
@@ -123,17 +147,21 @@ struct PlanHost {
 // LDS bytes of one element-stage slot of the widest variant that will run it; tiles are
 // grown while their LDS layout stays <= lds_budget and the kernel's per-thread limits
 // for a workgroup of `block` threads hold. checkBounds (3N, ela) may be null.
+// pair_window: cross-tile pairing between tiles on the same XCD at most this many
+// dispatch slots apart (<= 0: pairs within a tile only).
 int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, const int *row,
                const int *col, const int *checkBounds, const double *coord, int lds_budget,
-               int max_rows, int slot_bytes, int block);
+               int max_rows, int slot_bytes, int block, int pair_window);
 
 // Checks every index the kernel will follow against the array it indexes, that every
-// CSR entry is covered exactly once and that every entry's mask selects exactly the
-// elements holding both its nodes. Returns 0 or a negative status.
+// CSR entry is written exactly once (diagonal, task or transpose) and that every task's
+// mask selects exactly the elements holding both its nodes. Returns 0 or a negative
+// status.
 int validate_plan(const PlanHost &p, const int *row, const int *col, const int *e2n,
                   int slot_bytes);
 
-// Per-thread register limits of the kernel for a workgroup of `block` threads.
-inline int elems_per_thread(int block) { return kMaxTileElems / block; }
+// Per-thread register limits of the kernel for a workgroup of `block` threads: one
+// staged element per thread.
+inline int elems_per_thread(int) { return 1; }
 
 }  // namespace mf

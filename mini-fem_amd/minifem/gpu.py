@@ -201,12 +201,13 @@ class Plan:
     (mf_plan_create). assemble() runs one fused iteration: values + inverted prec."""
 
     def __init__(self, mesh, tile_lds_bytes=0, block_threads=0, max_tile_rows=0,
-                 use_coord=True):
+                 use_coord=True, pair_window=0):
         from ._lib import PlanParams
         _ensure()
         self._lib = lib()
         self.mesh = mesh
-        params = PlanParams(int(tile_lds_bytes), int(block_threads), int(max_tile_rows))
+        params = PlanParams(int(tile_lds_bytes), int(block_threads), int(max_tile_rows),
+                            int(pair_window))
         p = ctypes.c_void_p()
         check(self._lib.mf_plan_create(
             ctypes.byref(p), mesh.operatorID, mesh.nbElem, mesh.nbNodes, mesh.nnz,
@@ -218,7 +219,7 @@ class Plan:
     def stats(self):
         out = np.zeros(8, np.int64)
         check(self._lib.mf_plan_stats(self.ptr, out.ctypes.data), "mf_plan_stats")
-        keys = ["tiles", "tile_elems", "entries", "incidences", "plan_bytes", "lds_bytes",
+        keys = ["tiles", "tile_elems", "tasks", "incidences", "plan_bytes", "lds_bytes",
                 "block_threads", "max_rows"]
         return dict(zip(keys, (int(x) for x in out)))
 

@@ -12,6 +12,8 @@ from minifem.setup import plan_host_stats
 @pytest.mark.parametrize("params", [
     dict(), dict(tile_lds_bytes=8192), dict(tile_lds_bytes=150 * 1024, max_tile_rows=254),
     dict(max_tile_rows=1), dict(use_coord=False), dict(block_threads=512),
+    dict(block_threads=1024), dict(pair_window=4), dict(pair_window=10 ** 6),
+    dict(pair_window=-1),
 ])
 def test_plan_builds_and_validates(mf, n, shuffle, op, params):
     coord, e2n, code, nnz = mf.generate_cube(n, 0.2, 3, shuffle)
@@ -19,7 +21,16 @@ def test_plan_builds_and_validates(mf, n, shuffle, op, params):
     s = plan_host_stats(mesh, **params)
     assert s["tiles"] >= 1
     assert s["tile_elems"] >= mesh.nbElem  # every element is staged at least once
-    assert s["entries"] == mesh.nnz        # one entry lane per CSR block
+    # one task per off-diagonal pair computed once: at most every off-diagonal block, at
+    # least half of them (in-tile pairs always share a task)
+    off = mesh.nnz - mesh.nbNodes
+    assert (off + 1) // 2 <= s["tasks"] <= off
+    if s["tiles"] == 1:
+        assert s["tasks"] == off // 2  # every pair inside the one tile: computed once
+    if params.get("pair_window", 0) > 0:
+        # pairing across tiles only removes tasks
+        alone = plan_host_stats(mesh, **dict(params, pair_window=-1))
+        assert s["tasks"] <= alone["tasks"]
     # every (row, incident element) pair once; each row's list padded to 4, each tile's
     # to 8
     assert 4 * mesh.nbElem <= s["incidences"] <= 4 * mesh.nbElem + 3 * mesh.nbNodes + 4 * s["tiles"]
@@ -32,7 +43,8 @@ def test_plan_rejects_bad_parameters(mf):
     coord, e2n, code, nnz = mf.generate_cube(4, 0.2, 3, 0)
     mesh = mf.Mesh(coord, e2n, code, nnz, 1)
     for bad in (dict(tile_lds_bytes=100), dict(tile_lds_bytes=10 ** 6),
-                dict(max_tile_rows=300), dict(block_threads=128), dict(block_threads=384)):
+                dict(max_tile_rows=300), dict(block_threads=128), dict(block_threads=256),
+                dict(block_threads=384)):
         with pytest.raises(mf.MiniFemError):
             plan_host_stats(mesh, **bad)
 
@@ -79,4 +91,4 @@ def test_plan_accepts_valence_between_32_and_64(mf):
     nnz = 2 * (n_fan + 2) + 5 * n_fan
     mesh = mf.Mesh(coord.ravel(), e2n, code, nnz, 0)
     s = plan_host_stats(mesh)
-    assert s["entries"] == mesh.nnz
+    assert s["tasks"] <= mesh.nnz - mesh.nbNodes
