@@ -50,9 +50,12 @@ def main():
     ap.add_argument("--op", default="ela", choices=["ela", "lap"])
     ap.add_argument("--n", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tile-lds", type=int, default=0)
-    ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--max-rows", type=int, default=0)
+    # schedule of the tiled plan, as tuned for the 100^3 ela mesh on MI355X (the library's
+    # own defaults suit smaller meshes; minifem.tune picks per mesh)
+    ap.add_argument("--tile-lds", type=int, default=150000)
+    ap.add_argument("--block", type=int, default=1024)
+    ap.add_argument("--max-rows", type=int, default=254)
+    ap.add_argument("--pair-window", type=int, default=1 << 30)
     ap.add_argument("--copy", type=int, default=0,
                     help="instead of the FEM step, time the plain HBM copy kernel on this many "
                          "bytes (bandwidth reference and byte-counter calibration)")
@@ -100,7 +103,7 @@ def main():
     if args.schedule == "tiled":
         t1 = time.time()
         plan = gpu.Plan(mesh, tile_lds_bytes=args.tile_lds, block_threads=args.block,
-                        max_tile_rows=args.max_rows)
+                        max_tile_rows=args.max_rows, pair_window=args.pair_window)
         log(f"plan: {plan.stats()} ({time.time() - t1:.1f}s)")
     stream = gpu.Stream()
 
@@ -160,7 +163,7 @@ def main():
                "kind": "port",
                "sample": f"1 full iteration (assembly + prec_init + prec_inversion, REF order, "
                          f"serial, -O2) of the same {args.n}^3 {args.op} mesh: "
-                         f"{cpu_s:.2f} s; phases us {[round(x, 1) for x in tus]}",
+                         f"{cpu_s:.2f} s; phases us {[round(float(x), 1) for x in tus]}",
                "gpu_speedup": round(value / (mesh.nnz / cpu_s), 1)}
 
     out = {

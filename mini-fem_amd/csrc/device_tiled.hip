@@ -16,7 +16,7 @@
 
 struct mf_plan {
     int op = 0, nE = 0, N = 0, nnz = 0, dim = 1;
-    int ntiles = 0, block = 256, mask_words = 1;
+    int ntiles = 0, block = 512;
     int lds_fp64 = 0, lds_mixed = 0;
     int n_isolated = 0;
     int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -26,9 +26,9 @@ struct mf_plan {
     int32_t *tile_nodes = nullptr;
     int4 *tile_rowinfo = nullptr;
     uint16_t *inc = nullptr;
-    uint32_t *task_mask = nullptr;
     uint32_t *task_info = nullptr;
     int32_t *task_xpos = nullptr;
+    uint4 *task_rec = nullptr;
     int32_t *cb = nullptr;
     int32_t *isolated = nullptr;
 };
@@ -50,23 +50,6 @@ struct Vec2<float> {
     using type = float2;
 };
 
-template <int MW>
-struct MaskT;
-template <>
-struct MaskT<1> {
-    using type = uint32_t;
-};
-template <>
-struct MaskT<2> {
-    using type = uint64_t;
-};
-
-template <typename M>
-__device__ __forceinline__ int lowest_bit(M m)
-{
-    if constexpr (sizeof(M) == 8) return __builtin_ctzll(m);
-    else return __builtin_ctz(m);
-}
 
 // One tile element's gradient of vertex v from its stage slot: {x, y, z}.
 template <typename T, typename T2>
@@ -89,20 +72,19 @@ __device__ __forceinline__ void stage_grad(const T *sl, int v, T (&g)[3])
 // gathers after the first block of B has been computed and before that block's stores
 // (gfx950's vmcnt retires loads and stores in order, so a load issued behind stores waits
 // for them).
-template <int OP, int PREC, int BT, int MW>
+template <int OP, int PREC, int BT>
 __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
     k_tiled(const mf::TileHdr *__restrict__ tiles, int ntiles, int G,
             const int32_t *__restrict__ tile_elems, const uint32_t *__restrict__ tile_verts,
             const int32_t *__restrict__ tile_nodes, const int4 *__restrict__ rowinfo,
-            const uint4 *__restrict__ inc8, const uint32_t *__restrict__ task_mask,
-            const uint32_t *__restrict__ task_info, const int32_t *__restrict__ task_xpos,
+            const uint4 *__restrict__ inc8, const uint32_t *__restrict__ task_info,
+            const int32_t *__restrict__ task_xpos, const uint4 *__restrict__ task_rec,
             const double *__restrict__ coord,
             const double *__restrict__ coef, double *__restrict__ values,
             double *__restrict__ prec, int *__restrict__ singular)
 {
     using T = typename std::conditional<PREC == MF_PREC_FP64, double, float>::type;
     using T2 = typename Vec2<T>::type;
-    using M = typename MaskT<MW>::type;
     constexpr bool ELA = OP == MF_OP_ELA;
     constexpr int S = ELA ? mf::kStageEla : mf::kStageLap;
     constexpr int NA = ELA ? 9 : 1;
@@ -111,6 +93,7 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
     constexpr int NPT = (mf::kMaxTileNodes + BT - 1) / BT;     // tile nodes per thread
     constexpr int IPT = (mf::kMaxTileInc / 8 + BT - 1) / BT;   // 16-byte incidence loads
     constexpr int PB = mf::kMaxTileBlocks / NW;                // task blocks per wave
+    constexpr int TPT = mf::kMaxTileBlocks * 64 / BT;          // task records per thread
     extern __shared__ __align__(16) unsigned char lds_raw[];
 
     const int per = (ntiles + kXcd - 1) / kXcd;
@@ -124,9 +107,9 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
     uint32_t vert[EPT];
     int4 rinf[NPT];
     uint4 incv[IPT];
+    uint4 recv[TPT];
     uint32_t pinf[PB];
     int32_t pxp[PB];
-    M pm[PB];
     double cx[NPT][3], kc0[EPT], kc1[EPT];
 
     auto load_static = [&](const mf::TileHdr &hh) {
@@ -147,6 +130,11 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
             const int i = tid + r * BT;
             incv[r] = i < hh.n_inc / 8 ? inc8[hh.inc_start / 8 + i] : make_uint4(0, 0, 0, 0);
         }
+#pragma unroll
+        for (int r = 0; r < TPT; r++) {
+            const int i = tid + r * BT;
+            recv[r] = i < hh.n_task ? task_rec[hh.task_start + i] : make_uint4(0, 0, 0, 0);
+        }
         int dg, m0, mstep;
         mf::task_blocks(hh.n_rows, hh.n_task, NW, wave, dg, m0, mstep);
 #pragma unroll
@@ -154,17 +142,10 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
             const int idx = (m0 + k * mstep) * 64 + lane;
             pinf[k] = kNoEntry;
             pxp[k] = -1;
-            pm[k] = 0;
             if (m0 >= 0 && idx < hh.n_task) {
                 const int64_t e = (int64_t)hh.task_start + idx;
                 pinf[k] = task_info[e];
                 pxp[k] = task_xpos[e];
-                if constexpr (MW == 1) {
-                    pm[k] = task_mask[e];
-                } else {
-                    const uint2 w = reinterpret_cast<const uint2 *>(task_mask)[e];
-                    pm[k] = (uint64_t)w.x | (uint64_t)w.y << 32;
-                }
             }
         }
     };
@@ -202,9 +183,11 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
     load_gathers(h);
     for (;;) {
         const int NN = h.n_node, TE = h.n_elem, R = h.n_rows;
-        const mf::TileLds L = mf::tile_lds_layout(NN, TE, R, h.n_inc, S * (int)sizeof(T));
+        const mf::TileLds L =
+            mf::tile_lds_layout(NN, TE, R, h.n_inc, h.n_task, S * (int)sizeof(T));
         T *stage = reinterpret_cast<T *>(lds_raw + L.stage);
         int4 *lri = reinterpret_cast<int4 *>(lds_raw + L.rinfo);
+        uint4 *ltask = reinterpret_cast<uint4 *>(lds_raw + L.tasks);
         double *lc = reinterpret_cast<double *>(lds_raw + L.coords);
         int *lrnode = reinterpret_cast<int *>(lds_raw + L.rnode);
         uint32_t *lverts = reinterpret_cast<uint32_t *>(lds_raw + L.verts);
@@ -233,6 +216,11 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
         for (int r = 0; r < EPT; r++) {
             const int t = tid + r * BT;
             if (t < TE) lverts[t] = vert[r];
+        }
+#pragma unroll
+        for (int r = 0; r < TPT; r++) {
+            const int i = tid + r * BT;
+            if (i < h.n_task) ltask[i] = recv[r];
         }
         // the zero slot (index TE): the target of an unpaired second contribution
         if (tid < S) stage[TE * S + tid] = (T)0;
@@ -270,12 +258,10 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
         // this tile's task words, before the registers take the next tile's
         uint32_t tinf[PB];
         int32_t txp[PB];
-        M tm[PB];
 #pragma unroll
         for (int k = 0; k < PB; k++) {
             tinf[k] = pinf[k];
             txp[k] = pxp[k];
-            tm[k] = pm[k];
         }
         int dg, m0, mstep;
         mf::task_blocks(R, h.n_task, NW, wave, dg, m0, mstep);
@@ -379,50 +365,37 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
             }
         }
 
-        // ---- B2: off-diagonal tasks, one lane per block pair: the elements of the row's
-        // list that hold the column, two per iteration (independent LDS chains, summed in
-        // order); the block goes to row a, its transpose to row c
+        // ---- B2: off-diagonal tasks, one lane per block pair: the elements holding both a
+        // and c, two per iteration (independent LDS chains, summed in order); the block
+        // goes to row a, its transpose to row c
 #pragma clang loop unroll(disable)
         for (int k = 0; k < PB; k++) {
             const int m = m0 + k * mstep;
             if (m0 < 0 || m >= nEB) break;
             const uint32_t inf = tinf[0];
             const int32_t xp = txp[0];
-            M msk = tm[0];
 #pragma unroll
             for (int q = 0; q + 1 < PB; q++) {
                 tinf[q] = tinf[q + 1];
                 txp[q] = txp[q + 1];
-                tm[q] = tm[q + 1];
             }
             const bool on = inf != kNoEntry;  // lanes past the tile's last task idle
-            if (!on) msk = 0;
-            const int ra = inf & 0xFF, cl = (inf >> 8) & 0xFF, oa = inf >> 16;
+            const int ra = inf & 0xFF, cnt = on ? (inf >> 8) & 0xFF : 0, oa = inf >> 16;
             const int4 ri = on ? lri[ra] : make_int4(0, 0, 0, 0);
-            const int incOff = ri.z & 0xFFFF;
+            const uint4 rec = on ? ltask[m * 64 + lane] : make_uint4(0, 0, 0, 0);
             double acc[NA];
 #pragma unroll
             for (int q = 0; q < NA; q++) acc[q] = 0.0;
-            while (msk) {
-                // two contributions per iteration; a missing second one reads the
-                // all-zero slot TE, whose products are +0.0 and leave the sums unchanged
-                const int b0 = lowest_bit(msk);
-                msk &= msk - 1;
-                const bool two = msk != 0;
-                const int b1 = two ? lowest_bit(msk) : b0;
-                if (two) msk &= msk - 1;
-                const int e0 = linc[incOff + b0], e1 = linc[incOff + b1];
-                const int t[2] = {e0 >> 2, two ? e1 >> 2 : TE}, a[2] = {e0 & 3, e1 & 3};
+            // one contribution pair (element slot, vertex slots of a and c); a padding
+            // entry reads the all-zero slot TE, whose products are +0.0 and leave the
+            // sums unchanged
+            auto add_pair = [&](const int (&t)[2], const int (&a)[2], const int (&kk)[2]) {
                 T ga[2][3], gk[2][3], c0[2], c1[2], c2[2];
 #pragma unroll
                 for (int u = 0; u < 2; u++) {
-                    const uint32_t vv = lverts[t[u]];
-                    const int kk = (int)(vv & 0xFF) == cl ? 0
-                                   : (int)((vv >> 8) & 0xFF) == cl ? 1
-                                   : (int)((vv >> 16) & 0xFF) == cl ? 2 : 3;
                     const T *sl = stage + t[u] * S;
                     stage_grad<T, T2>(sl, a[u], ga[u]);
-                    stage_grad<T, T2>(sl, kk, gk[u]);
+                    stage_grad<T, T2>(sl, kk[u], gk[u]);
                     if constexpr (ELA) {
                         const T2 c01 = *reinterpret_cast<const T2 *>(sl + 12);
                         c0[u] = c01.x;
@@ -443,6 +416,39 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                         const T x = c0[u] * mf::lap_entry(ga[u], gk[u]);
                         acc[0] += (double)x;
                     }
+                }
+            };
+            if (cnt <= mf::kListLen) {
+                const uint32_t w[4] = {rec.x, rec.y, rec.z, rec.w};
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if (2 * u >= cnt) break;
+                    const int e0 = (int)(w[u] & 0xFFFF), e1 = (int)(w[u] >> 16);
+                    const int t[2] = {e0 >> 4, e1 >> 4}, a[2] = {(e0 >> 2) & 3, (e1 >> 2) & 3},
+                              kk[2] = {e0 & 3, e1 & 3};
+                    add_pair(t, a, kk);
+                }
+            } else {
+                // more than kListLen elements: walk the mask over row a's incidence list
+                uint64_t msk = (uint64_t)rec.x | (uint64_t)rec.y << 32;
+                const int cl = (int)rec.z, incOff = ri.z & 0xFFFF;
+                while (msk) {
+                    const int b0 = __builtin_ctzll(msk);
+                    msk &= msk - 1;
+                    const bool two = msk != 0;
+                    const int b1 = two ? __builtin_ctzll(msk) : b0;
+                    if (two) msk &= msk - 1;
+                    const int e0 = linc[incOff + b0], e1 = linc[incOff + b1];
+                    const int t[2] = {e0 >> 2, two ? e1 >> 2 : TE}, a[2] = {e0 & 3, e1 & 3};
+                    int kk[2];
+#pragma unroll
+                    for (int u = 0; u < 2; u++) {
+                        const uint32_t vv = lverts[t[u]];
+                        kk[u] = (int)(vv & 0xFF) == cl ? 0
+                                : (int)((vv >> 8) & 0xFF) == cl ? 1
+                                : (int)((vv >> 16) & 0xFF) == cl ? 2 : 3;
+                    }
+                    add_pair(t, a, kk);
                 }
             }
             if (pending) {
@@ -514,20 +520,20 @@ int upload_as(D **dst, const std::vector<S> &src)
 void free_plan(mf_plan *p)
 {
     if (!p) return;
-    void *ptrs[] = {p->tiles,     p->tile_elems, p->tile_verts, p->tile_nodes,
-                    p->tile_rowinfo, p->inc,     p->task_mask,  p->task_info,
-                    p->task_xpos, p->cb,         p->isolated};
+    void *ptrs[] = {p->tiles, p->tile_elems, p->tile_verts, p->tile_nodes, p->tile_rowinfo,
+                    p->inc,   p->task_info,  p->task_xpos,  p->task_rec,   p->cb,
+                    p->isolated};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     delete p;
 }
 
-template <int OP, int PREC, int BT, int MW>
+template <int OP, int PREC, int BT>
 int launch_tiled(mf_plan *p, const double *coord, const double *coef, double *values,
                  double *prec, int *singular, hipStream_t s)
 {
     const int lds = PREC == MF_PREC_FP64 ? p->lds_fp64 : p->lds_mixed;
-    auto kern = k_tiled<OP, PREC, BT, MW>;
+    auto kern = k_tiled<OP, PREC, BT>;
     if (hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                             hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
         return MF_ERR_HIP;
@@ -544,8 +550,8 @@ int launch_tiled(mf_plan *p, const double *coord, const double *coef, double *va
         const int G = std::max(1, std::min(per, std::max(1, per_cu) * std::max(1, cus / kXcd)));
         hipLaunchKernelGGL(kern, dim3(G * kXcd), dim3(BT), lds, s, p->tiles, p->ntiles, G,
                            p->tile_elems, p->tile_verts, p->tile_nodes, p->tile_rowinfo,
-                           reinterpret_cast<const uint4 *>(p->inc), p->task_mask, p->task_info,
-                           p->task_xpos, coord, coef, values, prec, singular);
+                           reinterpret_cast<const uint4 *>(p->inc), p->task_info, p->task_xpos,
+                           p->task_rec, coord, coef, values, prec, singular);
         MF_CHECK_LAUNCH();
     }
     if (p->n_isolated > 0) {
@@ -560,14 +566,9 @@ template <int OP, int PREC>
 int launch_variant(mf_plan *p, const double *coord, const double *coef, double *values,
                    double *prec, int *singular, hipStream_t s)
 {
-    const bool wide = p->mask_words == 2;
     switch (p->block) {
-    case 1024:
-        return wide ? launch_tiled<OP, PREC, 1024, 2>(p, coord, coef, values, prec, singular, s)
-                    : launch_tiled<OP, PREC, 1024, 1>(p, coord, coef, values, prec, singular, s);
-    case 512:
-        return wide ? launch_tiled<OP, PREC, 512, 2>(p, coord, coef, values, prec, singular, s)
-                    : launch_tiled<OP, PREC, 512, 1>(p, coord, coef, values, prec, singular, s);
+    case 512: return launch_tiled<OP, PREC, 512>(p, coord, coef, values, prec, singular, s);
+    case 1024: return launch_tiled<OP, PREC, 1024>(p, coord, coef, values, prec, singular, s);
     default: return MF_ERR_ARG;
     }
 }
@@ -608,8 +609,10 @@ int build_checked(mf::PlanHost &h, int op, int nE, int N, int nnz, const int *e2
     if (st != MF_OK) return st;
     int m64 = 0, m32 = 0;
     for (const auto &t : h.tiles) {
-        m64 = std::max(m64, mf::tile_lds_layout(t.n_node, t.n_elem, t.n_rows, t.n_inc, r.slot64).total);
-        m32 = std::max(m32, mf::tile_lds_layout(t.n_node, t.n_elem, t.n_rows, t.n_inc, r.slot32).total);
+        m64 = std::max(m64, mf::tile_lds_layout(t.n_node, t.n_elem, t.n_rows, t.n_inc, t.n_task,
+                                                r.slot64).total);
+        m32 = std::max(m32, mf::tile_lds_layout(t.n_node, t.n_elem, t.n_rows, t.n_inc, t.n_task,
+                                                r.slot32).total);
     }
     if (m64 + 16 > mf::kMaxLdsBytes) return MF_ERR_ARG;  // one row needs more than a CU has
     *lds64 = m64 + 16;
@@ -628,7 +631,7 @@ void fill_stats(const mf::PlanHost &h, int lds64, int block, int64_t out[8])
     out[4] = (int64_t)h.tiles.size() * sizeof(mf::TileHdr) + (int64_t)h.tile_elems.size() * 4 +
              (int64_t)h.tile_verts.size() + (int64_t)h.tile_nodes.size() * 4 +
              (int64_t)h.tile_rowinfo.size() * sizeof(mf::RowInfo) + (int64_t)h.inc.size() * 2 +
-             (int64_t)h.task_mask.size() * 4 + (int64_t)h.task_info.size() * 4 +
+             (int64_t)h.task_rec.size() * 4 + (int64_t)h.task_info.size() * 4 +
              (int64_t)h.task_xpos.size() * 4;
     out[5] = lds64;
     out[6] = block;
@@ -669,7 +672,6 @@ extern "C" int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nb
     p->dim = operatorID == MF_OP_ELA ? 9 : 1;
     p->ntiles = (int)h.tiles.size();
     p->block = r.block;
-    p->mask_words = h.mask_words;
     p->lds_fp64 = lds64;
     p->lds_mixed = lds32;
     p->n_isolated = (int)h.isolated.size();
@@ -679,7 +681,7 @@ extern "C" int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nb
     if (rc == MF_OK) rc = upload(&p->tile_nodes, h.tile_nodes);
     if (rc == MF_OK) rc = upload_as(&p->tile_rowinfo, h.tile_rowinfo);
     if (rc == MF_OK) rc = upload(&p->inc, h.inc);
-    if (rc == MF_OK) rc = upload(&p->task_mask, h.task_mask);
+    if (rc == MF_OK) rc = upload_as(&p->task_rec, h.task_rec);
     if (rc == MF_OK) rc = upload(&p->task_info, h.task_info);
     if (rc == MF_OK) rc = upload(&p->task_xpos, h.task_xpos);
     if (rc == MF_OK) rc = upload(&p->isolated, h.isolated);

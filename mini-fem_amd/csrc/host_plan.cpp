@@ -69,7 +69,7 @@ struct Partitioner {
         const int64_t inc8 = (inc + 7) / 8 * 8;
         if (inc8 > kMaxTileInc) return false;
         if (diag_blocks(ela, R) + (ent + kLanes - 1) / kLanes > kMaxTileBlocks) return false;
-        return tile_lds_layout((int)NN, (int)TE, R, (int)inc8, slot).total <= budget;
+        return tile_lds_layout((int)NN, (int)TE, R, (int)inc8, (int)ent, slot).total <= budget;
     }
 
     void rcb(int lo, int hi)
@@ -170,7 +170,6 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
         n2eI[i + 1] += n2eI[i];
     }
     if (maxVal > kMaxValence) return MF_ERR_ARG;
-    p.mask_words = maxVal > 32 ? 2 : 1;
     {
         std::vector<int> cur(n2eI.begin(), n2eI.end() - 1);
         for (int e = 0; e < nE; e++)
@@ -199,7 +198,6 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
 
     std::vector<int> nodeLocal((size_t)N, -1), elemLocal((size_t)nE, -1);
     std::vector<int> rows, tel;
-    const int MW = p.mask_words;
     for (int ti = 0; ti < T; ti++) {
         const auto &lf = part.leaves[ti];
         rows.assign(part.order.begin() + lf.first, part.order.begin() + lf.second);
@@ -284,15 +282,35 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
                     std::abs(tj - ti) <= pair_window)
                     return MF_ERR_ARG;  // the pattern is not symmetric
                 if (tj == ti && xpos < 0) return MF_ERR_ARG;
+                // the elements holding a and c, ascending, with a's and c's slots
                 uint64_t m = 0;
+                uint16_t list[kListLen];
+                int cnt = 0;
                 for (int j = n2eI[a]; j < n2eI[a + 1]; j++) {
                     const int *v = e2n + (int64_t)n2eV[j] * 4;
-                    if (v[0] - 1 == c || v[1] - 1 == c || v[2] - 1 == c || v[3] - 1 == c)
-                        m |= 1ull << (j - n2eI[a]);
+                    int va = -1, vc = -1;
+                    for (int k = 0; k < 4; k++) {
+                        if (v[k] - 1 == a) va = k;
+                        if (v[k] - 1 == c) vc = k;
+                    }
+                    if (vc < 0) continue;
+                    m |= 1ull << (j - n2eI[a]);
+                    if (cnt < kListLen) list[cnt] = (uint16_t)(elemLocal[n2eV[j]] << 4 | va << 2 | vc);
+                    cnt++;
                 }
-                p.task_info.push_back((uint32_t)(ra | nodeLocal[c] << 8 | o << 16));
-                for (int w = 0; w < MW; w++) p.task_mask.push_back((uint32_t)(m >> (32 * w)));
+                if (cnt < 1 || cnt > 0xFF) return MF_ERR_ARG;
+                p.task_info.push_back((uint32_t)(ra | cnt << 8 | o << 16));
                 p.task_xpos.push_back((int32_t)xpos);
+                if (cnt <= kListLen) {
+                    for (int q = cnt; q < kListLen; q++) list[q] = (uint16_t)(TE << 4);
+                    for (int w = 0; w < 4; w++)
+                        p.task_rec.push_back((uint32_t)list[2 * w] | (uint32_t)list[2 * w + 1] << 16);
+                } else {
+                    p.task_rec.push_back((uint32_t)m);
+                    p.task_rec.push_back((uint32_t)(m >> 32));
+                    p.task_rec.push_back((uint32_t)nodeLocal[c]);
+                    p.task_rec.push_back(0u);
+                }
             }
             while (p.inc.size() % kIncAlign) p.inc.push_back(0);
             incOff = (int)p.inc.size() - h.inc_start;
@@ -316,10 +334,8 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
                   int slot_bytes)
 {
     const bool ela = p.op == MF_OP_ELA;
-    const int MW = p.mask_words;
-    if (MW < 1 || MW > 2) return MF_ERR_STATE;
     if (p.tile_verts.size() != p.tile_elems.size() * 4) return MF_ERR_STATE;
-    if (p.task_mask.size() != p.task_info.size() * MW || p.task_xpos.size() != p.task_info.size())
+    if (p.task_rec.size() != p.task_info.size() * 4 || p.task_xpos.size() != p.task_info.size())
         return MF_ERR_STATE;
     std::vector<uint8_t> hit((size_t)p.nnz, 0), rowSeen((size_t)p.N, 0);
     for (int i : p.isolated) {
@@ -327,7 +343,7 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
         rowSeen[i] = 1;
     }
     const int NW = p.block / kLanes;
-    std::vector<int> nodeStamp((size_t)p.N, -1), cnt(kMaxTileNodes);
+    std::vector<int> nodeStamp((size_t)p.N, -1), localOf((size_t)p.N, -1), cnt(kMaxTileNodes);
     std::vector<uint64_t> holds(kMaxTileNodes, 0);
     int tileNo = 0;
     for (const TileHdr &h : p.tiles) {
@@ -361,7 +377,7 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
             for (int i = 0; i < nDB; i++)
                 if (dseen[i] != 1) return MF_ERR_STATE;
         }
-        if (tile_lds_layout(h.n_node, h.n_elem, R, h.n_inc, slot_bytes).total + 16 > kMaxLdsBytes)
+        if (tile_lds_layout(h.n_node, h.n_elem, R, h.n_inc, h.n_task, slot_bytes).total + 16 > kMaxLdsBytes)
             return MF_ERR_STATE;
         const int32_t *nodes = p.tile_nodes.data() + h.node_start;
         const uint8_t *verts = p.tile_verts.data() + (size_t)h.elem_start * 4;
@@ -370,6 +386,7 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
             const int n = nodes[i];
             if (n < 0 || n >= p.N || nodeStamp[n] == tileNo) return MF_ERR_STATE;
             nodeStamp[n] = tileNo;
+            localOf[n] = i;
             if (i < R) {
                 if (rowSeen[n]) return MF_ERR_STATE;
                 rowSeen[n] = 1;
@@ -398,7 +415,7 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
             if (ri.rowptr != row[a] || len != row[a + 1] - row[a] || len < 1 || doff >= len ||
                 col[row[a] + doff] != a + 1 || mask > 7 || (!ela && mask))
                 return MF_ERR_STATE;
-            if (incOff != incNext || incOff % kIncAlign || val < 1 || val > 32 * MW ||
+            if (incOff != incNext || incOff % kIncAlign || val < 1 || val > kMaxValence ||
                 incOff + val > h.n_inc)
                 return MF_ERR_STATE;
             incNext += (val + kIncAlign - 1) / kIncAlign * kIncAlign;
@@ -416,14 +433,35 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
             // the row's tasks: exact masks, each block and its transpose written once
             for (; tk < tkEnd && (int)(p.task_info[tk] & 0xFF) == ra; tk++) {
                 const uint32_t inf = p.task_info[tk];
-                const int ec = (inf >> 8) & 0xFF, oa = inf >> 16;
-                if (ec >= h.n_node || ec == ra || oa >= len) return MF_ERR_STATE;
-                const int c = nodes[ec];
-                if (col[row[a] + oa] != c + 1) return MF_ERR_STATE;
-                if (hit[(size_t)row[a] + oa]++) return MF_ERR_STATE;
-                uint64_t m = p.task_mask[(size_t)tk * MW];
-                if (MW == 2) m |= (uint64_t)p.task_mask[(size_t)tk * MW + 1] << 32;
-                if (m != holds[ec] || m == 0) return MF_ERR_STATE;
+                const int cnt = (inf >> 8) & 0xFF, oa = inf >> 16;
+                if (oa >= len) return MF_ERR_STATE;
+                const int c = col[row[a] + oa] - 1;
+                if (c == a || hit[(size_t)row[a] + oa]++) return MF_ERR_STATE;
+                const uint32_t *rec = p.task_rec.data() + (size_t)tk * 4;
+                const int ec = nodeStamp[c] == tileNo ? localOf[c] : -1;
+                if (ec < 0 || __builtin_popcountll(holds[ec]) != cnt || cnt < 1) return MF_ERR_STATE;
+                if (cnt <= kListLen) {
+                    // the exact ascending list of (element, slot of a, slot of c), padded
+                    // with the zero slot
+                    int q = 0;
+                    for (int i = 0; i < val; i++) {
+                        if (!((holds[ec] >> i) & 1)) continue;
+                        const uint16_t en = p.inc[(size_t)h.inc_start + incOff + i];
+                        const int t = en >> 2;
+                        int vc = -1;
+                        for (int k = 0; k < 4; k++)
+                            if (verts[t * 4 + k] == ec) vc = k;
+                        const uint16_t want = (uint16_t)(t << 4 | (en & 3) << 2 | vc);
+                        if (((rec[q / 2] >> (16 * (q % 2))) & 0xFFFF) != want) return MF_ERR_STATE;
+                        q++;
+                    }
+                    for (; q < kListLen; q++)
+                        if (((rec[q / 2] >> (16 * (q % 2))) & 0xFFFF) != (uint32_t)(h.n_elem << 4))
+                            return MF_ERR_STATE;
+                } else {
+                    const uint64_t m = (uint64_t)rec[0] | (uint64_t)rec[1] << 32;
+                    if (m != holds[ec] || (int)rec[2] != ec || rec[3] != 0) return MF_ERR_STATE;
+                }
                 const int64_t xp = p.task_xpos[tk];
                 if (xp >= 0) {
                     if (xp < row[c] || xp >= row[c + 1] || col[xp] != a + 1 || hit[xp]++)

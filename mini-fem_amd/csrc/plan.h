@@ -9,8 +9,9 @@
 //      (color-permuted) element order -- the summation order of the reference's
 //      COLORING build, src/assembly.cc:246-271 -- and writes it once:
 //      - a row's incident elements are kept in LDS as an ascending "incidence list";
-//      - an off-diagonal block (row a, column c) is one lane, which walks the bits of a
-//        precomputed mask over row a's incidence list (the elements that also hold c);
+//      - an off-diagonal block (row a, column c) is one lane, which reads the elements
+//        holding both a and c, with a's and c's vertex slots in them, from its task
+//        record (or, past 8 elements, walks a mask over row a's incidence list);
 //        block (c,a) is its exact transpose (same elements in the same order, and
 //        IEEE products commute), so a pair is computed once and written twice when
 //        c is a row of the same tile, or of a tile that runs close in time on the same
@@ -42,6 +43,7 @@ constexpr int kMaxValence = 64;          // incident elements per row (mask widt
 constexpr int kMaxLdsBytes = 160 * 1024; // LDS of one gfx950 CU
 constexpr int kIncAlign = 4;             // row incidence lists start at multiples of 4
 constexpr int kXcds = 8;                 // MI355X: workgroups are dealt to 8 XCDs round-robin
+constexpr int kListLen = 8;              // contributions held in one task record
 
 // Element-stage slot of one tile element, in units of the stage type (double or float):
 // the (x,y) gradient pairs of vertices 0..3 at 2i (16-byte aligned for double2 reads),
@@ -56,20 +58,21 @@ constexpr int kStageLap = 14;
 MF_HOST_DEVICE inline int align16(int x) { return (x + 15) & ~15; }
 
 // LDS of one tile: [element stage: n_elem + 1 slots, the last one all zero] [row info:
-// n_rows x 16 B] [node coordinates: n_node x 24 B] [row node ids: n_rows x 4 B]
-// [element vertices: n_elem + 1 x 4 B] [incidence lists: n_inc x 2 B, n_inc a multiple
-// of 8].
+// n_rows x 16 B] [task records: n_task x 16 B] [node coordinates: n_node x 24 B] [row
+// node ids: n_rows x 4 B] [element vertices: n_elem + 1 x 4 B] [incidence lists: n_inc x
+// 2 B, n_inc a multiple of 8].
 struct TileLds {
-    int stage, rinfo, coords, rnode, verts, inc, total;
+    int stage, rinfo, tasks, coords, rnode, verts, inc, total;
 };
 
 MF_HOST_DEVICE inline TileLds tile_lds_layout(int n_node, int n_elem, int n_rows, int n_inc,
-                                              int slot_bytes)
+                                              int n_task, int slot_bytes)
 {
     TileLds L;
     L.stage = 0;
     L.rinfo = align16((n_elem + 1) * slot_bytes);
-    L.coords = L.rinfo + n_rows * 16;
+    L.tasks = L.rinfo + n_rows * 16;
+    L.coords = L.tasks + n_task * 16;
     L.rnode = L.coords + n_node * 24;
     L.verts = L.rnode + n_rows * 4;
     L.inc = align16(L.verts + (n_elem + 1) * 4);
@@ -100,7 +103,7 @@ struct TileHdr {
     int32_t n_rows;
     int32_t rinfo_start;         // into tile_rowinfo: n_rows entries
     int32_t inc_start, n_inc;    // into inc (multiples of 8)
-    int32_t task_start, n_task;  // into task_mask / task_info / task_xpos
+    int32_t task_start, n_task;  // into task_info / task_xpos / task_rec
     int32_t pad[2];
 };
 
@@ -121,23 +124,25 @@ struct RowInfo {
 // Each row's list starts at a multiple of kIncAlign entries (zero padding between lists:
 // element 0, read but never summed), so a lane reads 4 entries with one 8-byte access.
 // Off-diagonal tasks of a tile, in tile-row order and CSR order inside a row:
-//   task_info[e] = ra | c << 8 | oa << 16  (tile row, tile-local column node, offset of
-//                  the block in row ra);
-//   task_mask[e * mask_words + w]: bit b set when the b-th element of row ra's
-//                  incidence list holds node c;
+//   task_info[e] = ra | n << 8 | oa << 16  (tile row, number of elements holding both a
+//                  and c, offset of the block in row ra);
 //   task_xpos[e]:  CSR position of block (c,a), written with the transpose, or -1 when
-//                  c's own tile computes it.
+//                  c's own tile computes it;
+//   task_rec[e]:   16 bytes. n <= kListLen: 8 contributions t << 4 | a << 2 | k (tile
+//                  element, slots of a and c in it) ascending, the unused ones t = n_elem
+//                  (the zero slot). n > kListLen: {mask low, mask high, c, 0}, bit b of
+//                  the mask set when the b-th element of row ra's incidence list holds c.
 struct PlanHost {
-    int op = 0, nE = 0, N = 0, nnz = 0, block = 256, mask_words = 1;
+    int op = 0, nE = 0, N = 0, nnz = 0, block = 512;
     std::vector<TileHdr> tiles;
     std::vector<int32_t> tile_elems;   // global (permuted) element ids, ascending per tile
     std::vector<uint8_t> tile_verts;   // 4 tile-local node ids per tile element
     std::vector<int32_t> tile_nodes;   // global node ids (0-based)
     std::vector<RowInfo> tile_rowinfo; // per tile row
     std::vector<uint16_t> inc;
-    std::vector<uint32_t> task_mask;
     std::vector<uint32_t> task_info;
     std::vector<int32_t> task_xpos;
+    std::vector<uint32_t> task_rec;    // 4 words per task
     std::vector<int32_t> isolated;     // nodes with an empty row (no element)
     int max_node = 0, max_elem = 0, max_rows = 0;
 };
