@@ -155,6 +155,25 @@ int mf_event_destroy(void *event);
 int mf_event_record(void *event, void *stream);
 int mf_event_elapsed_ms(void *start, void *stop, float *ms);
 
+/* Pinned host memory (hipHostMalloc), asynchronous copies (the host buffer must be
+ * pinned for them to overlap), event polling (done = 1 when complete) and cross-stream
+ * ordering: the request loop's building blocks. */
+int mf_host_alloc(void **ptr, size_t bytes);
+int mf_host_free(void *ptr);
+int mf_memcpy_h2d_async(void *dst, const void *src, size_t bytes, void *stream);
+int mf_memcpy_d2h_async(void *dst, const void *src, size_t bytes, void *stream);
+int mf_event_query(void *event, int *done);
+int mf_stream_wait_event(void *stream, void *event);
+
+/* Stream capture into an executable HIP graph: mf_graph_begin(stream), enqueue work
+ * (e.g. one mf_plan_assemble per mesh of a batch) on that stream, mf_graph_end(stream,
+ * &graph); mf_graph_launch replays the whole batch as one submission. The captured
+ * kernels read and write the buffers they were given at capture time. */
+int mf_graph_begin(void *stream);
+int mf_graph_end(void *stream, void **graph);
+int mf_graph_launch(void *graph, void *stream);
+int mf_graph_destroy(void *graph);
+
 /* Plain streaming device-to-device copy kernel (16-byte aligned, size a multiple of
  * 16): the HBM bandwidth reference for rooflines and byte-counter calibration. */
 int mf_hbm_copy(void *dst, const void *src, size_t bytes, void *stream);

@@ -19,6 +19,7 @@ struct mf_plan {
     int ntiles = 0, block = 512;
     int lds_fp64 = 0, lds_mixed = 0;
     int n_isolated = 0;
+    int grid[2] = {0, 0};  // persistent grid per precision, set on its first launch
     int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     mf::TileHdr *tiles = nullptr;
     int32_t *tile_elems = nullptr;
@@ -281,6 +282,9 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
         // ---- B1: diagonal blocks, one lane per row: the whole incidence list, ascending,
         // four entries per LDS read with their stage reads issued together
         if (dg >= 0) {
+            // the diagonal chains are the tile's critical path: this wave's instructions
+            // go first on its SIMD while it sums them
+            __builtin_amdgcn_s_setprio(3);
             const int r = dg * 64 + lane;
             const bool on = r < R;
             const int4 ri = on ? lri[r] : make_int4(0, 0, 0, 0);
@@ -364,6 +368,8 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                 }
             }
         }
+
+        __builtin_amdgcn_s_setprio(0);
 
         // ---- B2: off-diagonal tasks, one lane per block pair: the elements holding both a
         // and c, two per iteration (independent LDS chains, summed in order); the block
@@ -534,20 +540,25 @@ int launch_tiled(mf_plan *p, const double *coord, const double *coef, double *va
 {
     const int lds = PREC == MF_PREC_FP64 ? p->lds_fp64 : p->lds_mixed;
     auto kern = k_tiled<OP, PREC, BT>;
-    if (hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
-        return MF_ERR_HIP;
     if (p->ntiles > 0) {
-        // persistent grid: as many workgroups per XCD as fit at once, at most one per
-        // slot of its tile range
-        int dev = 0, cus = 0, per_cu = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(kern),
-                                                         BT, lds) != hipSuccess)
-            return MF_ERR_HIP;
-        const int per = (p->ntiles + kXcd - 1) / kXcd;
-        const int G = std::max(1, std::min(per, std::max(1, per_cu) * std::max(1, cus / kXcd)));
+        int &G = p->grid[PREC == MF_PREC_FP64 ? 0 : 1];
+        if (G == 0) {
+            // first launch: allow the whole LDS for this kernel (the attribute is per
+            // kernel, shared by every plan), then the persistent grid = as many workgroups
+            // per XCD as fit at once, at most one per slot of its tile range. Later
+            // launches (also under graph capture) make no runtime queries.
+            int dev = 0, cus = 0, per_cu = 0;
+            if (hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    mf::kMaxLdsBytes) != hipSuccess ||
+                hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per_cu, reinterpret_cast<const void *>(kern), BT, lds) != hipSuccess)
+                return MF_ERR_HIP;
+            const int per = (p->ntiles + kXcd - 1) / kXcd;
+            G = std::max(1, std::min(per, std::max(1, per_cu) * std::max(1, cus / kXcd)));
+        }
         hipLaunchKernelGGL(kern, dim3(G * kXcd), dim3(BT), lds, s, p->tiles, p->ntiles, G,
                            p->tile_elems, p->tile_verts, p->tile_nodes, p->tile_rowinfo,
                            reinterpret_cast<const uint4 *>(p->inc), p->task_info, p->task_xpos,

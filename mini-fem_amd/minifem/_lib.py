@@ -72,6 +72,16 @@ SIGNATURES = {
     "mf_event_record": (_i, [_p, _p]),
     "mf_event_elapsed_ms": (_i, [_p, _p, _p]),
     "mf_hbm_copy": (_i, [_p, _p, ctypes.c_size_t, _p]),
+    "mf_host_alloc": (_i, [_p, ctypes.c_size_t]),
+    "mf_host_free": (_i, [_p]),
+    "mf_memcpy_h2d_async": (_i, [_p, _p, ctypes.c_size_t, _p]),
+    "mf_memcpy_d2h_async": (_i, [_p, _p, ctypes.c_size_t, _p]),
+    "mf_event_query": (_i, [_p, _p]),
+    "mf_stream_wait_event": (_i, [_p, _p]),
+    "mf_graph_begin": (_i, [_p]),
+    "mf_graph_end": (_i, [_p, _p]),
+    "mf_graph_launch": (_i, [_p, _p]),
+    "mf_graph_destroy": (_i, [_p]),
     "mf_plan_create": (_i, [_p, _i, _i, _i, _i, _p, _p, _p, _p, _p, _p]),
     "mf_plan_destroy": (_i, [_p]),
     "mf_plan_stats": (_i, [_p, _p]),

@@ -86,9 +86,64 @@ def _sp(stream):
 def _p(a):
     if a is None:
         return None
-    if isinstance(a, DeviceArray):
+    if isinstance(a, (DeviceArray, DeviceView)):
         return a.ptr
     raise TypeError(f"expected a DeviceArray, got {type(a)}")
+
+
+class DeviceView:
+    """A typed window [offset, offset + n) into device memory another object owns (an
+    arena sub-allocation): same interface as DeviceArray, never freed by itself."""
+
+    def __init__(self, ptr, n, dtype, owner=None):
+        self.ptr = int(ptr)
+        self.size = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.size * self.dtype.itemsize
+        self._owner = owner
+
+    def numpy(self, stream=None):
+        out = np.empty(self.size, self.dtype)
+        check(lib().mf_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes, _sp(stream)), "d2h")
+        return out
+
+    def fill(self, value, stream=None):
+        a = np.full(self.size, value, self.dtype)
+        check(lib().mf_memcpy_h2d(self.ptr, a.ctypes.data, self.nbytes, _sp(stream)), "h2d")
+
+
+class HostArray:
+    """Pinned host memory (mf_host_alloc) as a numpy array, for asynchronous copies."""
+
+    def __init__(self, n, dtype):
+        _ensure()
+        self.dtype = np.dtype(dtype)
+        self.size = int(n)
+        self.nbytes = self.size * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(lib().mf_host_alloc(ctypes.byref(p), max(self.nbytes, 1)), "mf_host_alloc")
+        self.ptr = p.value
+        self._lib = lib()
+        buf = (ctypes.c_byte * max(self.nbytes, 1)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=self.dtype, count=self.size)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self.array = None
+            self._lib.mf_host_free(self.ptr)
+            self.ptr = None
+
+
+def copy_h2d_async(dst, src, stream):
+    """dst: DeviceArray/DeviceView; src: HostArray (pinned), same byte size."""
+    assert dst.nbytes == src.nbytes
+    check(lib().mf_memcpy_h2d_async(dst.ptr, src.ptr, src.nbytes, _sp(stream)), "h2d async")
+
+
+def copy_d2h_async(dst, src, stream):
+    """dst: HostArray (pinned); src: DeviceArray/DeviceView, same byte size."""
+    assert dst.nbytes == src.nbytes
+    check(lib().mf_memcpy_d2h_async(dst.ptr, src.ptr, src.nbytes, _sp(stream)), "d2h async")
 
 
 class Stream:
@@ -101,6 +156,10 @@ class Stream:
 
     def sync(self):
         check(self._lib.mf_stream_sync(self.ptr), "mf_stream_sync")
+
+    def wait(self, event):
+        """Work enqueued on this stream after the call waits for `event`."""
+        check(self._lib.mf_stream_wait_event(self.ptr, event.ptr), "mf_stream_wait_event")
 
     def __del__(self):
         if getattr(self, "ptr", None):
@@ -119,6 +178,12 @@ class Event:
     def record(self, stream=None):
         check(self._lib.mf_event_record(self.ptr, _sp(stream)), "mf_event_record")
 
+    def query(self):
+        """True once the work recorded before the event has completed."""
+        done = ctypes.c_int()
+        check(self._lib.mf_event_query(self.ptr, ctypes.byref(done)), "mf_event_query")
+        return bool(done.value)
+
     def elapsed_ms(self, end):
         ms = ctypes.c_float()
         check(self._lib.mf_event_elapsed_ms(self.ptr, end.ptr, ctypes.byref(ms)), "elapsed")
@@ -132,6 +197,31 @@ class Event:
 
 def synchronize():
     check(lib().mf_device_sync(), "mf_device_sync")
+
+
+class Graph:
+    """Work captured from a stream into an executable HIP graph, replayed as one
+    submission (mf_graph_begin / mf_graph_end / mf_graph_launch)."""
+
+    def __init__(self, stream, enqueue):
+        """Captures everything `enqueue(stream)` puts on `stream`."""
+        self._lib = lib()
+        check(self._lib.mf_graph_begin(_sp(stream)), "mf_graph_begin")
+        try:
+            enqueue(stream)
+        finally:
+            p = ctypes.c_void_p()
+            st = self._lib.mf_graph_end(_sp(stream), ctypes.byref(p))
+        check(st, "mf_graph_end")
+        self.ptr = p.value
+
+    def launch(self, stream):
+        check(self._lib.mf_graph_launch(self.ptr, _sp(stream)), "mf_graph_launch")
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self._lib.mf_graph_destroy(self.ptr)
+            self.ptr = None
 
 
 def assembly(coord, values, row, col, elemToNode, elemToEdge, nbElem, nbEdges, operatorDim,

@@ -163,7 +163,8 @@ class Mesh:
     elements permuted by color, CSR built from the permuted elements, Dirichlet mask.
 
     Attributes: nbElem, nbNodes, nnz, coord, elemToNode (permuted), coef (permuted or
-    None), colorPerm, colorToElem, row, col, checkBounds, operatorID, dim.
+    None), colorPerm, colorToElem, row, col, checkBounds, operatorID, dim;
+    elemToNode_original / coef_original keep the input element order.
     """
 
     def __init__(self, coord, elemToNode, boundNodesCode, nbEdges, operator_id, coef=None,
@@ -172,6 +173,8 @@ class Mesh:
         self.dim = 9 if self.operatorID == OP_ELA else 1
         self.coord = _c(coord, np.float64)
         e2n = _c(elemToNode, np.int32)
+        self.elemToNode_original = e2n
+        self.coef_original = None if coef is None else _c(coef, np.float64)
         self.nbElem = e2n.size // 4
         self.nbNodes = self.coord.size // 3
         if colored:
@@ -192,6 +195,18 @@ class Mesh:
     @property
     def nbColors(self):
         return len(self.colorToElem) - 1
+
+    def permute_elements(self, per_element):
+        """A per-element array (lap: nE values; ela: nE (lambda, mu) pairs) given in the
+        original element order, moved to this mesh's color-permuted order."""
+        a = _c(per_element, np.float64)
+        return permute_2d(a, self.colorPerm, a.size // self.nbElem)
+
+    def unpermute_elements(self, per_element):
+        """The inverse of permute_elements: color-permuted order back to the original."""
+        a = _c(per_element, np.float64)
+        w = a.size // self.nbElem
+        return np.ascontiguousarray(a.reshape(self.nbElem, w)[self.colorPerm].ravel())
 
     @classmethod
     def cube(cls, n, jitter, geom_seed, operator_id, shuffle_seed=0, coef_seed=None,
