@@ -128,6 +128,19 @@ int mf_boundary_mask(const int *boundNodesCode, int nbNodes, int nbBoundNodes,
 /* compute_double_norm (src/FEM.cc:48-56), host, serial order. */
 double mf_compute_double_norm(const double *tab, int64_t size);
 
+/* Device twins of the setup above, bit-exact with them, for meshes that are already
+ * resident on the GPU (all arrays device memory; each call returns after its work is
+ * done). Greedy first-fit coloring is a sequential scan and stays on the host.
+ *   mf_dev_create_nodeToElem: index[N+1], value[4nE], each node's elements ascending.
+ *   mf_dev_create_nodeToNode: row[N+1], col[nbEdges]; returns nnz or MF_ERR_NNZ.
+ *   mf_dev_create_elemToEdge: elemToEdge[16 nE]. */
+int mf_dev_create_nodeToElem(const int *elemToNode, int nbElem, int nbNodes, int *index,
+                             int *value, void *stream);
+int mf_dev_create_nodeToNode(int *row, int *col, const int *n2eIndex, const int *n2eValue,
+                             const int *elemToNode, int nbNodes, int nbEdges, void *stream);
+int mf_dev_create_elemToEdge(const int *row, const int *col, const int *elemToNode,
+                             int *elemToEdge, int nbElem, void *stream);
+
 /* ---------------------------------------------------------------------------------
  * GPU: drop-in twins of the per-iteration entry points (device pointers).
  * ------------------------------------------------------------------------------- */

@@ -255,6 +255,29 @@ def norm(x, stream=None):
     return float(out[0])
 
 
+def device_setup(elemToNode, nbNodes, nbEdges, stream=None):
+    """nodeToElem, the CSR pattern and elemToEdge built on the GPU from a (color-permuted,
+    1-based) elemToNode: bit-exact twins of the host setup. Returns DeviceArrays
+    (index, value, row, col, elemToEdge)."""
+    _ensure()
+    e2n = elemToNode if isinstance(elemToNode, (DeviceArray, DeviceView)) else \
+        DeviceArray.from_numpy(np.asarray(elemToNode, np.int32))
+    nE = e2n.size // 4
+    index = DeviceArray(nbNodes + 1, np.int32)
+    value = DeviceArray(max(4 * nE, 1), np.int32)
+    check(lib().mf_dev_create_nodeToElem(e2n.ptr, nE, int(nbNodes), index.ptr, value.ptr,
+                                         _sp(stream)), "mf_dev_create_nodeToElem")
+    row = DeviceArray(nbNodes + 1, np.int32)
+    col = DeviceArray(max(int(nbEdges), 1), np.int32)
+    nnz = check(lib().mf_dev_create_nodeToNode(row.ptr, col.ptr, index.ptr, value.ptr, e2n.ptr,
+                                               int(nbNodes), int(nbEdges), _sp(stream)),
+                "mf_dev_create_nodeToNode")
+    e2e = DeviceArray(max(16 * nE, 1), np.int32)
+    check(lib().mf_dev_create_elemToEdge(row.ptr, col.ptr, e2n.ptr, e2e.ptr, nE, _sp(stream)),
+          "mf_dev_create_elemToEdge")
+    return index, value, row, col, e2e, nnz
+
+
 class DeviceMesh:
     """A Mesh's arrays resident on the GPU, with value/prec buffers."""
 

@@ -1,0 +1,101 @@
+"""The batched graph and the serving loop on the GPU.
+
+Batched: the captured batch graph, replayed several times with the program's own eager
+work and synchronisations in between, reproduces the eager per-mesh results bit for bit
+on every replay; one member against the oracle; mixed precision within 1e-6 of FP64.
+
+Serving: every request's (values, prec) served in the mix -- overlapping requests on
+several streams under an HBM budget that forces admission to wait -- equals the same
+request served alone, bit for bit; one served request against the oracle with that
+request's input.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def small_batch(mf, count=8):
+    from minifem.batched import batch_meshes
+    return batch_meshes(count=count, n_lo=3000, n_hi=30000, seed=1000)
+
+
+def test_batched_graph_replays_equal_eager(mf, orc, gpu):
+    from minifem.batched import Batch
+    print("[batched] graph vs eager", flush=True)
+    meshes = small_batch(mf)
+    b = Batch(meshes, branches=4)
+    b.run_eager()
+    eager = b.results()
+    b.capture()
+    for rep in range(3):
+        for dm in b.dms:  # poison: every replay must write every block again
+            dm.values.fill(np.nan)
+            dm.prec.fill(np.nan)
+        b.launch()
+        b.stream.sync()
+        got = b.results()
+        for (v0, p0), (v1, p1) in zip(eager, got):
+            assert np.array_equal(v0, v1) and np.array_equal(p0, p1), f"replay {rep}"
+        b.run_eager()  # eager work between replays
+        gpu.synchronize()
+    # one member against the oracle
+    m = meshes[1]
+    om = orc.OracleMesh(m.coord, m.elemToNode_original, m.boundNodesCode, m.nnz, m.operatorID,
+                        coef=m.coef_original)
+    ov = om.assemble()
+    assert np.array_equal(eager[1][0], ov)
+    assert np.array_equal(eager[1][1], om.prec(ov)[0])
+
+
+def test_batched_mixed_precision_error(mf, gpu):
+    from minifem.batched import Batch
+    print("[batched] mixed", flush=True)
+    meshes = small_batch(mf, count=4)
+    b64 = Batch(meshes, branches=2)
+    b64.run_eager()
+    bmx = Batch(meshes, precision=mf.PREC_MIXED, branches=2).capture()
+    bmx.launch()
+    bmx.stream.sync()
+    for (v0, p0), (v1, _) in zip(b64.results(), bmx.results()):
+        rel = np.linalg.norm(v1 - v0) / np.linalg.norm(v0)
+        assert rel <= 1e-6, rel
+
+
+def test_serving_mix_equals_alone(mf, orc, gpu):
+    from minifem.serve import Server, make_requests, served_meshes
+    print("[serve] mix vs alone", flush=True)
+    meshes = served_meshes(count=6, n_lo=1000, n_hi=60000, seed=1000)
+    reqs = make_requests(meshes, count=40, rate_hz=20000.0, seed=5000)
+    fp = max(8 * m.dim * (m.nnz + m.nbNodes) + 8 * max(m.coord.size, m.coef.size)
+             for m in meshes)
+    srv = Server(meshes, hbm_budget=int(2.2 * fp), slots=4)
+    srv.stage(reqs)
+    stats, mixed = srv.run(reqs, keep_results=True)
+    assert stats["requests"] == len(reqs)
+    assert 0 < stats["p50_ms"] <= stats["p99_ms"]
+    assert srv.arena.used == 0
+    for r in reqs:
+        v, p = srv.run_alone(r)
+        assert np.array_equal(v, mixed[r.rid][0]), r.rid
+        assert np.array_equal(p, mixed[r.rid][1]), r.rid
+    # a coefficient request and a coordinate request against the oracle
+    for r in reqs[:2]:
+        m = meshes[r.mesh]
+        coord = r.payload if r.kind == "coord" else m.coord
+        coef = m.coef_original if r.kind == "coord" else m.unpermute_elements(r.payload)
+        om = orc.OracleMesh(coord, m.elemToNode_original, m.boundNodesCode, m.nnz, m.operatorID,
+                            coef=coef)
+        ov = om.assemble()
+        assert np.array_equal(mixed[r.rid][0], ov)
+        assert np.array_equal(mixed[r.rid][1], om.prec(ov)[0])
+
+
+def test_serving_rejects_request_above_budget(mf, gpu):
+    from minifem.serve import Server, make_requests, served_meshes
+    meshes = served_meshes(count=2, n_lo=2000, n_hi=3000, seed=1000)
+    reqs = make_requests(meshes, count=2, rate_hz=1000.0)
+    srv = Server(meshes, hbm_budget=4096, slots=1)
+    srv.stage(reqs)
+    with pytest.raises(ValueError):
+        srv.run(reqs)
