@@ -53,6 +53,17 @@ struct Vec2<float> {
 
 
 // One tile element's gradient of vertex v from its stage slot: {x, y, z}.
+// One 3x3 block (9 doubles, 8-byte aligned) as four 16-byte stores and one 8-byte
+// store: half the store instructions of nine 8-byte ones, twice the bytes per request
+// (gfx950 global stores need only dword alignment).
+__device__ __forceinline__ void store_block9(double *dst, const double (&b)[9])
+{
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        *reinterpret_cast<double2 *>(dst + 2 * q) = make_double2(b[2 * q], b[2 * q + 1]);
+    dst[8] = b[8];
+}
+
 template <typename T, typename T2>
 __device__ __forceinline__ void stage_grad(const T *sl, int v, T (&g)[3])
 {
@@ -327,15 +338,11 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                 }
                 double blk[9] = {s0, s1, s2, s1, s4, s5, s2, s5, s8};
                 if (on) {
-                    double *dv = values + dpos * 9;
-#pragma unroll
-                    for (int q = 0; q < 9; q++) dv[q] = blk[q];
+                    store_block9(values + dpos * 9, blk);
                 }
                 const int info = mf::ela_invert_block(blk, (ri.y >> 24) & 7, true);
                 if (on) {
-                    double *dp = prec + (int64_t)lrnode[r] * 9;
-#pragma unroll
-                    for (int q = 0; q < 9; q++) dp[q] = blk[q];
+                    store_block9(prec + (int64_t)lrnode[r] * 9, blk);
                     if (info != 0 && singular) atomicAdd(singular, 1);
                 }
             } else {
@@ -462,13 +469,16 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                 pending = false;
             }
             if (on) {
-                double *dst = values + ((int64_t)ri.x + oa) * NA;
-#pragma unroll
-                for (int q = 0; q < NA; q++) dst[q] = acc[q];
-                if (xp >= 0) {
-                    double *dx = values + (int64_t)xp * NA;
-#pragma unroll
-                    for (int q = 0; q < NA; q++) dx[q] = acc[(q % 3) * 3 + q / 3];  // transpose
+                if constexpr (ELA) {
+                    store_block9(values + ((int64_t)ri.x + oa) * 9, acc);
+                    if (xp >= 0) {
+                        const double tr[9] = {acc[0], acc[3], acc[6], acc[1], acc[4],
+                                              acc[7], acc[2], acc[5], acc[8]};  // transpose
+                        store_block9(values + (int64_t)xp * 9, tr);
+                    }
+                } else {
+                    values[(int64_t)ri.x + oa] = acc[0];
+                    if (xp >= 0) values[xp] = acc[0];
                 }
             }
         }
