@@ -223,6 +223,16 @@ int mf_prec_inversion(double *prec, const int *row, const int *col,
                       const int *checkBounds, int nbNodes, int operatorID,
                       int *singular, void *stream);
 
+/* Consumer of one iteration's outputs (the reference stops before its "solver & update",
+ * src/main.cc:353): block-Jacobi preconditioned CG on K = P A P + (I - P), A the
+ * assembled CSR operator (values, row, col as assembled), P zeroing the dofs that
+ * checkBounds masks (ela per dof; lap a node with any flag; NULL: none), M^-1 = prec as
+ * prec_inversion leaves it. Solves K x = P b from the initial x (device arrays of N*d
+ * doubles) to ||r|| <= rtol ||P b|| or maxIter iterations; reports both. */
+int mf_pcg(int operatorID, int nbNodes, const int *row, const int *col, const double *values,
+           const double *prec, const int *checkBounds, const double *b, double *x, double rtol,
+           int maxIter, int *iterations, double *relResidual, void *stream);
+
 /* compute_double_norm (src/FEM.cc:48-56) on the device: deterministic tree sum of
  * squares, sqrt on the host. x is device memory; blocks until the result is ready. */
 int mf_norm(const double *x, int64_t n, double *result, void *stream);

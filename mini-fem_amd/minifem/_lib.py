@@ -75,6 +75,7 @@ SIGNATURES = {
     "mf_dev_create_nodeToElem": (_i, [_p, _i, _i, _p, _p, _p]),
     "mf_dev_create_nodeToNode": (_i, [_p, _p, _p, _p, _p, _i, _i, _p]),
     "mf_dev_create_elemToEdge": (_i, [_p, _p, _p, _p, _i, _p]),
+    "mf_pcg": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p, ctypes.c_double, _i, _p, _p, _p]),
     "mf_host_alloc": (_i, [_p, ctypes.c_size_t]),
     "mf_host_free": (_i, [_p]),
     "mf_memcpy_h2d_async": (_i, [_p, _p, ctypes.c_size_t, _p]),

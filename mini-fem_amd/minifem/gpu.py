@@ -278,6 +278,19 @@ def device_setup(elemToNode, nbNodes, nbEdges, stream=None):
     return index, value, row, col, e2e, nnz
 
 
+def pcg(operator_id, nbNodes, row, col, values, prec, checkBounds, b, x, rtol=1e-10,
+        max_iter=1000, stream=None):
+    """Block-Jacobi preconditioned CG on the assembled operator (mf_pcg); x is updated in
+    place. Returns (iterations, relative residual)."""
+    _ensure()
+    it = ctypes.c_int()
+    rr = ctypes.c_double()
+    check(lib().mf_pcg(int(operator_id), int(nbNodes), _p(row), _p(col), _p(values), _p(prec),
+                       _p(checkBounds), _p(b), _p(x), float(rtol), int(max_iter),
+                       ctypes.byref(it), ctypes.byref(rr), _sp(stream)), "mf_pcg")
+    return it.value, rr.value
+
+
 class DeviceMesh:
     """A Mesh's arrays resident on the GPU, with value/prec buffers."""
 
