@@ -52,10 +52,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     # schedule of the tiled plan, as tuned for the 100^3 ela mesh on MI355X (the library's
     # own defaults suit smaller meshes; minifem.tune picks per mesh)
-    ap.add_argument("--tile-lds", type=int, default=150000)
+    ap.add_argument("--tile-lds", type=int, default=155000)
     ap.add_argument("--block", type=int, default=1024)
     ap.add_argument("--max-rows", type=int, default=254)
-    ap.add_argument("--pair-window", type=int, default=1 << 30)
     ap.add_argument("--batched", action="store_true",
                     help="instead: 64 meshes (5k-200k nodes, lap/ela) in one captured graph")
     ap.add_argument("--serve", action="store_true",
@@ -114,7 +113,7 @@ def main():
     if args.schedule == "tiled":
         t1 = time.time()
         plan = gpu.Plan(mesh, tile_lds_bytes=args.tile_lds, block_threads=args.block,
-                        max_tile_rows=args.max_rows, pair_window=args.pair_window)
+                        max_tile_rows=args.max_rows)
         log(f"plan: {plan.stats()} ({time.time() - t1:.1f}s)")
     stream = gpu.Stream()
 

@@ -249,14 +249,12 @@ typedef struct mf_plan mf_plan;
 
 /* Tuning knobs of the tiled schedule (0 = default). An off-diagonal block (c,a) is
  * the exact transpose of (a,c) (same elements, same order, commutative products), so
- * the plan computes each pair once and writes both: always when a and c are rows of
- * one tile; across tiles that run on the same XCD at most pair_window dispatch slots
- * apart when pair_window > 0 (< 0: within tiles only). */
+ * when a and c are rows of one tile the plan computes the pair once and writes both;
+ * a tile computes every block of its rows itself, so each tile writes whole CSR rows. */
 typedef struct mf_plan_params {
     int tile_lds_bytes;   /* LDS budget of one tile: 4096..163840 (default 65536) */
     int block_threads;    /* threads per workgroup: 512 or 1024 (default 512) */
     int max_tile_rows;    /* cap on rows per tile, <= 256 (default 64 ela, 128 lap) */
-    int pair_window;      /* cross-tile pairing window, see above */
 } mf_plan_params;
 
 /* Host arrays: elemToNode (color-permuted, 1-based, 4nE), row[N+1], col[nnz],

@@ -17,6 +17,7 @@
 struct mf_plan {
     int op = 0, nE = 0, N = 0, nnz = 0, dim = 1;
     int ntiles = 0, block = 512;
+    int max_node = 0, max_elem = 0;  // the largest tile: sizes the fixed front of its LDS
     int lds_fp64 = 0, lds_mixed = 0;
     int n_isolated = 0;
     int grid[2] = {0, 0};  // persistent grid per precision, set on its first launch
@@ -37,8 +38,6 @@ struct mf_plan {
 namespace {
 
 constexpr int kXcd = mf::kXcds;
-constexpr int kDefaultPairWindow = 0;  // cross-tile pairing off unless asked for
-constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
 
 template <typename T>
 struct Vec2;
@@ -51,8 +50,6 @@ struct Vec2<float> {
     using type = float2;
 };
 
-
-// One tile element's gradient of vertex v from its stage slot: {x, y, z}.
 // One 3x3 block (9 doubles, 8-byte aligned) as four 16-byte stores and one 8-byte
 // store: half the store instructions of nine 8-byte ones, twice the bytes per request
 // (gfx950 global stores need only dword alignment).
@@ -64,6 +61,7 @@ __device__ __forceinline__ void store_block9(double *dst, const double (&b)[9])
     dst[8] = b[8];
 }
 
+// One tile element's gradient of vertex v from its stage slot: {x, y, z}.
 template <typename T, typename T2>
 __device__ __forceinline__ void stage_grad(const T *sl, int v, T (&g)[3])
 {
@@ -78,15 +76,16 @@ __device__ __forceinline__ void stage_grad(const T *sl, int v, T (&g)[3])
 // an XCD walk its range together, slot s0, s0 + G, ..., so tiles that share elements and
 // nodes run on one XCD close in time and share its L2.
 //
-// Per tile: fill LDS from registers -> A (element stage) -> B (blocks, stores). The next
-// tile's global loads overlap the current tile: its static data (ids, vertex lists, row
-// info, incidence lists, task words) is issued after A, its coordinate and coefficient
-// gathers after the first block of B has been computed and before that block's stores
-// (gfx950's vmcnt retires loads and stores in order, so a load issued behind stores waits
-// for them).
+// Waves 0..NC-1 compute, waves NC.. load. Per tile, separated by four barriers:
+//   A  compute: element stage          | load: row data, tasks, incidences into LDS
+//   B  compute: blocks into registers  | load: next tile's static loads, then its gathers
+//   O  compute: blocks into the output image (laid over the stage)
+//   W  compute: image -> values, prec  | load: wait gathers, next vertices/coords/coefs
+// Compute waves issue no global loads and loader waves no stores, so no wave ever waits
+// for its own stores to drain.
 template <int OP, int PREC, int BT>
 __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
-    k_tiled(const mf::TileHdr *__restrict__ tiles, int ntiles, int G,
+    k_tiled(const mf::TileHdr *__restrict__ tiles, int ntiles, int G, int f_node, int f_elem,
             const int32_t *__restrict__ tile_elems, const uint32_t *__restrict__ tile_verts,
             const int32_t *__restrict__ tile_nodes, const int4 *__restrict__ rowinfo,
             const uint4 *__restrict__ inc8, const uint32_t *__restrict__ task_info,
@@ -100,12 +99,14 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
     constexpr bool ELA = OP == MF_OP_ELA;
     constexpr int S = ELA ? mf::kStageEla : mf::kStageLap;
     constexpr int NA = ELA ? 9 : 1;
-    constexpr int NW = BT / 64;
-    constexpr int EPT = 1;                                     // tile elements per thread
-    constexpr int NPT = (mf::kMaxTileNodes + BT - 1) / BT;     // tile nodes per thread
-    constexpr int IPT = (mf::kMaxTileInc / 8 + BT - 1) / BT;   // 16-byte incidence loads
-    constexpr int PB = mf::kMaxTileBlocks / NW;                // task blocks per wave
-    constexpr int TPT = mf::kMaxTileBlocks * 64 / BT;          // task records per thread
+    constexpr int NC = mf::compute_waves(BT), CT = NC * 64, LT = BT - CT;
+    constexpr int MAXE = mf::max_tile_elems(BT);
+    constexpr int EA = (MAXE + CT - 1) / CT;                              // A: elements per thread
+    constexpr int NPT = (mf::kMaxTileNodes + LT - 1) / LT;               // loader: nodes, rows
+    constexpr int EPT = (MAXE + LT - 1) / LT;                            // loader: elements
+    constexpr int IPT = (mf::kMaxTileInc / 8 + LT - 1) / LT;             // loader: 16-byte incidences
+    constexpr int TPT = (mf::max_tile_blocks(BT) * 64 + LT - 1) / LT;    // loader: tasks
+    static_assert(mf::kMaxTileRows <= mf::kMaxTileNodes, "rows are the first tile nodes");
     extern __shared__ __align__(16) unsigned char lds_raw[];
 
     const int per = (ntiles + kXcd - 1) / kXcd;
@@ -113,139 +114,204 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
     int slot = blockIdx.x / kXcd;
     if (slot >= per || xcd * per + slot >= ntiles) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-
-    // registers holding one tile's global data: static (round 1) and gathered (round 2)
-    int nodeId[NPT], elemId[EPT];
-    uint32_t vert[EPT];
-    int4 rinf[NPT];
-    uint4 incv[IPT];
-    uint4 recv[TPT];
-    uint32_t pinf[PB];
-    int32_t pxp[PB];
-    double cx[NPT][3], kc0[EPT], kc1[EPT];
-
-    auto load_static = [&](const mf::TileHdr &hh) {
-#pragma unroll
-        for (int r = 0; r < NPT; r++) {
-            const int i = tid + r * BT;
-            nodeId[r] = i < hh.n_node ? tile_nodes[hh.node_start + i] : 0;
-            rinf[r] = i < hh.n_rows ? rowinfo[hh.rinfo_start + i] : make_int4(0, 0, 0, 0);
-        }
-#pragma unroll
-        for (int r = 0; r < EPT; r++) {
-            const int t = tid + r * BT;
-            elemId[r] = t < hh.n_elem ? tile_elems[hh.elem_start + t] : 0;
-            vert[r] = t < hh.n_elem ? tile_verts[hh.elem_start + t] : 0u;
-        }
-#pragma unroll
-        for (int r = 0; r < IPT; r++) {
-            const int i = tid + r * BT;
-            incv[r] = i < hh.n_inc / 8 ? inc8[hh.inc_start / 8 + i] : make_uint4(0, 0, 0, 0);
-        }
-#pragma unroll
-        for (int r = 0; r < TPT; r++) {
-            const int i = tid + r * BT;
-            recv[r] = i < hh.n_task ? task_rec[hh.task_start + i] : make_uint4(0, 0, 0, 0);
-        }
-        int dg, m0, mstep;
-        mf::task_blocks(hh.n_rows, hh.n_task, NW, wave, dg, m0, mstep);
-#pragma unroll
-        for (int k = 0; k < PB; k++) {
-            const int idx = (m0 + k * mstep) * 64 + lane;
-            pinf[k] = kNoEntry;
-            pxp[k] = -1;
-            if (m0 >= 0 && idx < hh.n_task) {
-                const int64_t e = (int64_t)hh.task_start + idx;
-                pinf[k] = task_info[e];
-                pxp[k] = task_xpos[e];
-            }
-        }
+    auto layout = [&](const mf::TileHdr &hh) {
+        return mf::tile_lds_layout(ELA, f_node, f_elem, hh.n_elem, hh.n_rows, hh.n_inc,
+                                   hh.n_task, hh.n_ent, S * (int)sizeof(T));
     };
-    auto load_gathers = [&](const mf::TileHdr &hh) {
+    mf::TileHdr h = tiles[xcd * per + slot];
+
+    if (wave >= NC) {
+        // ------------------------------------------------------------------ loader waves
+        const int lt = tid - CT;
+        int nodeId[NPT];
+        double cx[NPT][3];
+        // 16-byte records as four scalar arrays each (an array of vectors stays in scratch)
+        int rix[NPT], riy[NPT], riz[NPT], riw[NPT];
+        int elemId[EPT];
+        uint32_t vert[EPT];
+        double k0[EPT], k1[EPT];  // raw coefficients: ela (lambda, mu), lap (k, unused)
+        uint32_t inx[IPT], iny[IPT], inz[IPT], inw[IPT];
+        uint32_t rcx[TPT], rcy[TPT], rcz[TPT], rcw[TPT];
+        uint32_t tinf[TPT];
+        int32_t txp[TPT];
+        // every load is unconditional, from an index clamped into the tile's range (the
+        // fill skips the lanes past it): no branch between the loads, so they all stay in
+        // flight together
+        auto at = [](int start, int n, int i) -> int64_t {
+            return n > 0 ? (int64_t)start + (i < n ? i : n - 1) : 0;
+        };
+        auto load_static = [&](const mf::TileHdr &hh) {
 #pragma unroll
-        for (int r = 0; r < NPT; r++) {
-            const int i = tid + r * BT;
-            cx[r][0] = cx[r][1] = cx[r][2] = 0.0;
-            if (i < hh.n_node) {
+            for (int r = 0; r < NPT; r++) {
+                const int i = lt + r * LT;
+                nodeId[r] = tile_nodes[at(hh.node_start, hh.n_node, i)];
+                const int4 v = rowinfo[at(hh.rinfo_start, hh.n_rows, i)];
+                rix[r] = v.x;
+                riy[r] = v.y;
+                riz[r] = v.z;
+                riw[r] = v.w;
+            }
+#pragma unroll
+            for (int r = 0; r < EPT; r++) {
+                const int i = lt + r * LT;
+                elemId[r] = tile_elems[at(hh.elem_start, hh.n_elem, i)];
+                vert[r] = tile_verts[at(hh.elem_start, hh.n_elem, i)];
+            }
+#pragma unroll
+            for (int r = 0; r < IPT; r++) {
+                const uint4 v = inc8[at(hh.inc_start / 8, hh.n_inc / 8, lt + r * LT)];
+                inx[r] = v.x;
+                iny[r] = v.y;
+                inz[r] = v.z;
+                inw[r] = v.w;
+            }
+#pragma unroll
+            for (int r = 0; r < TPT; r++) {
+                const int64_t e = at(hh.task_start, hh.n_task, lt + r * LT);
+                const uint4 v = task_rec[e];
+                rcx[r] = v.x;
+                rcy[r] = v.y;
+                rcz[r] = v.z;
+                rcw[r] = v.w;
+                tinf[r] = task_info[e];
+                txp[r] = task_xpos[e];
+            }
+        };
+        auto load_gathers = [&]() {
+#pragma unroll
+            for (int r = 0; r < NPT; r++) {
                 const double *p = coord + (int64_t)nodeId[r] * 3;
                 cx[r][0] = p[0];
                 cx[r][1] = p[1];
                 cx[r][2] = p[2];
             }
-        }
+            // without a coefficient array, a valid dummy address keeps the loads
+            // branch-free (fill1 substitutes the constants)
+            const bool hc = coef != nullptr;
 #pragma unroll
-        for (int r = 0; r < EPT; r++) {
-            const int t = tid + r * BT;
-            kc0[r] = ELA ? 0.25 : 1.0;  // reference constants: lambda 0.25, mu 1 | k 1
-            kc1[r] = 1.0;
-            if (t < hh.n_elem && coef) {
+            for (int r = 0; r < EPT; r++) {
                 if constexpr (ELA) {
-                    const double2 lm = reinterpret_cast<const double2 *>(coef)[elemId[r]];
-                    kc0[r] = lm.x;
-                    kc1[r] = lm.y;
+                    const double2 lm = reinterpret_cast<const double2 *>(hc ? coef : coord)[hc ? elemId[r] : 0];
+                    k0[r] = lm.x;
+                    k1[r] = lm.y;
                 } else {
-                    kc0[r] = coef[elemId[r]];
+                    k0[r] = (hc ? coef : coord)[hc ? elemId[r] : 0];
                 }
             }
-        }
-    };
+        };
+        // vertices, coordinates, coefficients: what A reads
+        auto fill1 = [&](const mf::TileHdr &hh) {
+            const mf::TileLds L = layout(hh);
+            double *lc = reinterpret_cast<double *>(lds_raw + L.coords);
+            uint32_t *lverts = reinterpret_cast<uint32_t *>(lds_raw + L.verts);
+#pragma unroll
+            for (int r = 0; r < NPT; r++) {
+                const int i = lt + r * LT;
+                if (i < hh.n_node) {
+                    lc[i * 3 + 0] = cx[r][0];
+                    lc[i * 3 + 1] = cx[r][1];
+                    lc[i * 3 + 2] = cx[r][2];
+                }
+            }
+            // without a coefficient array: the reference constants (lambda 0.25, mu 1 | k 1)
+            const bool hc = coef != nullptr;
+#pragma unroll
+            for (int r = 0; r < EPT; r++) {
+                const int i = lt + r * LT;
+                if (i < hh.n_elem) {
+                    lverts[i] = vert[r];
+                    if constexpr (ELA)
+                        reinterpret_cast<double2 *>(lds_raw + L.coef)[i] =
+                            make_double2(hc ? k0[r] : 0.25, hc ? k1[r] : 1.0);
+                    else
+                        reinterpret_cast<double *>(lds_raw + L.coef)[i] = hc ? k0[r] : 1.0;
+                }
+            }
+            if (lt == 0) lverts[hh.n_elem] = 0u;  // the zero slot's vertices
+        };
+        // row info, row nodes, incidence lists, tasks: what B, O and W read
+        auto fill2 = [&](const mf::TileHdr &hh) {
+            const mf::TileLds L = layout(hh);
+            int4 *lri = reinterpret_cast<int4 *>(lds_raw + L.rinfo);
+            int *lrnode = reinterpret_cast<int *>(lds_raw + L.rnode);
+#pragma unroll
+            for (int r = 0; r < NPT; r++) {
+                const int i = lt + r * LT;
+                if (i < hh.n_rows) {
+                    lri[i] = make_int4(rix[r], riy[r], riz[r], riw[r]);
+                    lrnode[i] = nodeId[r];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < IPT; r++) {
+                const int i = lt + r * LT;
+                if (i < hh.n_inc / 8)
+                    reinterpret_cast<uint4 *>(lds_raw + L.inc)[i] = make_uint4(inx[r], iny[r], inz[r], inw[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < TPT; r++) {
+                const int i = lt + r * LT;
+                if (i < hh.n_task) {
+                    reinterpret_cast<uint4 *>(lds_raw + L.tasks)[i] = make_uint4(rcx[r], rcy[r], rcz[r], rcw[r]);
+                    reinterpret_cast<uint2 *>(lds_raw + L.twords)[i] = make_uint2(tinf[r], (uint32_t)txp[r]);
+                }
+            }
+        };
 
-    mf::TileHdr h = tiles[xcd * per + slot];
-    load_static(h);
-    load_gathers(h);
+        load_static(h);
+        load_gathers();
+        fill1(h);
+        __syncthreads();  // P
+        for (;;) {
+            // A
+            fill2(h);
+            __syncthreads();  // 1
+            // B
+            const int nslot = slot + G;
+            const bool has_next = nslot < per && xcd * per + nslot < ntiles;
+            mf::TileHdr hn = h;
+            if (has_next) {
+                hn = tiles[xcd * per + nslot];
+                load_static(hn);
+                load_gathers();
+            }
+            __syncthreads();  // 2
+            // O
+            __syncthreads();  // 3
+            // W
+            if (has_next) fill1(hn);
+            __syncthreads();  // 4
+            if (!has_next) break;
+            h = hn;
+            slot = nslot;
+        }
+        return;
+    }
+
+    // ---------------------------------------------------------------------- compute waves
+    __syncthreads();  // P
     for (;;) {
         const int NN = h.n_node, TE = h.n_elem, R = h.n_rows;
-        const mf::TileLds L =
-            mf::tile_lds_layout(NN, TE, R, h.n_inc, h.n_task, S * (int)sizeof(T));
+        (void)NN;
+        const mf::TileLds L = layout(h);
         T *stage = reinterpret_cast<T *>(lds_raw + L.stage);
-        int4 *lri = reinterpret_cast<int4 *>(lds_raw + L.rinfo);
-        uint4 *ltask = reinterpret_cast<uint4 *>(lds_raw + L.tasks);
-        double *lc = reinterpret_cast<double *>(lds_raw + L.coords);
-        int *lrnode = reinterpret_cast<int *>(lds_raw + L.rnode);
-        uint32_t *lverts = reinterpret_cast<uint32_t *>(lds_raw + L.verts);
-        uint16_t *linc = reinterpret_cast<uint16_t *>(lds_raw + L.inc);
-
-        // ---- LDS fill
-#pragma unroll
-        for (int r = 0; r < NPT; r++) {
-            const int i = tid + r * BT;
-            if (i < R) {
-                lri[i] = rinf[r];
-                lrnode[i] = nodeId[r];
-            }
-            if (i < NN) {
-                lc[i * 3 + 0] = cx[r][0];
-                lc[i * 3 + 1] = cx[r][1];
-                lc[i * 3 + 2] = cx[r][2];
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < IPT; r++) {
-            const int i = tid + r * BT;
-            if (i < h.n_inc / 8) reinterpret_cast<uint4 *>(linc)[i] = incv[r];
-        }
-#pragma unroll
-        for (int r = 0; r < EPT; r++) {
-            const int t = tid + r * BT;
-            if (t < TE) lverts[t] = vert[r];
-        }
-#pragma unroll
-        for (int r = 0; r < TPT; r++) {
-            const int i = tid + r * BT;
-            if (i < h.n_task) ltask[i] = recv[r];
-        }
-        // the zero slot (index TE): the target of an unpaired second contribution
-        if (tid < S) stage[TE * S + tid] = (T)0;
-        if (tid == 0) lverts[TE] = 0u;
-        __syncthreads();
+        double *outv = reinterpret_cast<double *>(lds_raw + L.stage);  // the output image
+        const int4 *lri = reinterpret_cast<const int4 *>(lds_raw + L.rinfo);
+        const uint4 *ltask = reinterpret_cast<const uint4 *>(lds_raw + L.tasks);
+        const uint2 *ltw = reinterpret_cast<const uint2 *>(lds_raw + L.twords);
+        const double *lc = reinterpret_cast<const double *>(lds_raw + L.coords);
+        const int *lrnode = reinterpret_cast<const int *>(lds_raw + L.rnode);
+        const uint32_t *lverts = reinterpret_cast<const uint32_t *>(lds_raw + L.verts);
+        const uint16_t *linc = reinterpret_cast<const uint16_t *>(lds_raw + L.inc);
 
         // ---- A: scaled gradients + coefficients of every tile element, one slot each
 #pragma unroll
-        for (int r = 0; r < EPT; r++) {
-            const int t = tid + r * BT;
+        for (int k = 0; k < EA; k++) {
+            const int t = tid + k * CT;
             if (t >= TE) continue;
-            const int vi[4] = {(int)(vert[r] & 0xFF), (int)((vert[r] >> 8) & 0xFF),
-                               (int)((vert[r] >> 16) & 0xFF), (int)(vert[r] >> 24)};
+            const uint32_t vv = lverts[t];
+            const int vi[4] = {(int)(vv & 0xFF), (int)((vv >> 8) & 0xFF), (int)((vv >> 16) & 0xFF),
+                               (int)(vv >> 24)};
             double c[4][3], g[4][3];
 #pragma unroll
             for (int i = 0; i < 4; i++)
@@ -260,145 +326,123 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
             }
             if constexpr (ELA) {
                 // src/assembly.cc:386-409 with per-element (lambda, mu)
-                const double cd = kc0[r] + 2.0 * kc1[r], co = kc0[r] + kc1[r], cm = kc1[r];
+                const double2 lm = reinterpret_cast<const double2 *>(lds_raw + L.coef)[t];
+                const double cd = lm.x + 2.0 * lm.y, co = lm.x + lm.y, cm = lm.y;
                 *reinterpret_cast<T2 *>(sl + 12) = T2{(T)cd, (T)co};
                 *reinterpret_cast<T2 *>(sl + 14) = T2{(T)cm, (T)0};
             } else {
-                *reinterpret_cast<T2 *>(sl + 12) = T2{(T)kc0[r], (T)0};
+                const double kk = reinterpret_cast<const double *>(lds_raw + L.coef)[t];
+                *reinterpret_cast<T2 *>(sl + 12) = T2{(T)kk, (T)0};
             }
         }
-        // this tile's task words, before the registers take the next tile's
-        uint32_t tinf[PB];
-        int32_t txp[PB];
+        // the zero slot (index TE): the target of an unpaired second contribution
+        if (tid < S) stage[TE * S + tid] = (T)0;
+        __syncthreads();  // 1
+
+        // ---- B: one work block per compute wave, results held in registers
+        const int nDB = mf::diag_blocks(ELA, R), nEB = (h.n_task + 63) / 64;
+        const bool isDiag = wave < nDB;
+        const int r = wave * 64 + lane;          // diagonal: the lane's row
+        const bool dOn = isDiag && r < R;
+        int4 dri = make_int4(0, 0, 0, 0);
+        double blk[NA];
 #pragma unroll
-        for (int k = 0; k < PB; k++) {
-            tinf[k] = pinf[k];
-            txp[k] = pxp[k];
-        }
-        int dg, m0, mstep;
-        mf::task_blocks(R, h.n_task, NW, wave, dg, m0, mstep);
-        const int nEB = (h.n_task + 63) / 64;
-        __syncthreads();
-
-        // ---- the next tile's static data, in flight during B
-        const int nslot = slot + G;
-        const bool has_next = nslot < per && xcd * per + nslot < ntiles;
-        mf::TileHdr hn = h;
-        if (has_next) {
-            hn = tiles[xcd * per + nslot];
-            load_static(hn);
-        }
-        bool pending = has_next;  // the next tile's gathers are still to be issued
-
-        // ---- B1: diagonal blocks, one lane per row: the whole incidence list, ascending,
-        // four entries per LDS read with their stage reads issued together
-        if (dg >= 0) {
+        for (int q = 0; q < NA; q++) blk[q] = 0.0;
+        const int m = wave - nDB;                // task block
+        const int ti = m * 64 + lane;
+        const bool tOn = !isDiag && m < nEB && ti < h.n_task;
+        int tEnt = 0;                            // tile entry of the task's block
+        int32_t xp = -1;
+        if (isDiag) {
             // the diagonal chains are the tile's critical path: this wave's instructions
             // go first on its SIMD while it sums them
             __builtin_amdgcn_s_setprio(3);
-            const int r = dg * 64 + lane;
-            const bool on = r < R;
-            const int4 ri = on ? lri[r] : make_int4(0, 0, 0, 0);
-            const int incOff = ri.z & 0xFFFF, val = on ? (ri.z >> 16) & 0xFFFF : 0;
-            const int64_t dpos = (int64_t)ri.x + ((ri.y >> 16) & 0xFF);
-            if constexpr (ELA) {
-                // ela_block(g, g): exactly symmetric (g_d*g_e == g_e*g_d in IEEE), so six
-                // sums hold the block bit for bit
-                double s0 = 0.0, s1 = 0.0, s2 = 0.0, s4 = 0.0, s5 = 0.0, s8 = 0.0;
-                for (int i = 0; i < val; i += 4) {
-                    const uint2 e4 = *reinterpret_cast<const uint2 *>(linc + incOff + i);
-                    const int en[4] = {(int)(e4.x & 0xFFFF), (int)(e4.x >> 16),
-                                       (int)(e4.y & 0xFFFF), (int)(e4.y >> 16)};
-                    T g[4][3], cd[4], co[4], cm[4];
+            dri = dOn ? lri[r] : make_int4(0, 0, 0, 0);
+            const int incOff = dri.z & 0xFFFF, val = dOn ? (dri.z >> 16) & 0xFFFF : 0;
+            // The row's incidence list, two entries per 32-bit word (lists are padded to a
+            // multiple of 4 entries), summed in order one pair at a time. Software
+            // pipelined: the LDS reads of the next pair are in flight while the current
+            // pair is summed; reads past the list repeat its last word and are never
+            // summed.
+            const uint32_t *lw = reinterpret_cast<const uint32_t *>(linc + incOff);
+            const int np = (val + 1) >> 1, lastw = val > 0 ? ((val + 3) >> 2) * 2 - 1 : 0;
+            auto word = [&](int q) { return lw[q < lastw ? q : lastw]; };
+            struct Pair {
+                T g[2][3];
+                T c[2][3];  // ela: lambda + 2 mu, lambda + mu, mu; lap: k
+            };
+            auto fetch = [&](uint32_t w, Pair &P) {
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        const T *sl = stage + (en[u] >> 2) * S;
-                        stage_grad<T, T2>(sl, en[u] & 3, g[u]);
+                for (int u = 0; u < 2; u++) {
+                    const int e = u ? (int)(w >> 16) : (int)(w & 0xFFFF);
+                    const T *sl = stage + (e >> 2) * S;
+                    stage_grad<T, T2>(sl, e & 3, P.g[u]);
+                    if constexpr (ELA) {
                         const T2 c01 = *reinterpret_cast<const T2 *>(sl + 12);
-                        cd[u] = c01.x;
-                        co[u] = c01.y;
-                        cm[u] = sl[14];
+                        P.c[u][0] = c01.x;
+                        P.c[u][1] = c01.y;
+                        P.c[u][2] = sl[14];
+                    } else {
+                        P.c[u][0] = sl[12];
                     }
+                }
+            };
+            // ela_block(g, g) is exactly symmetric (g_d*g_e == g_e*g_d in IEEE): six
+            // sums hold the block bit for bit
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s4 = 0.0, s5 = 0.0, s8 = 0.0;
+            auto sum = [&](const Pair &P, int p) {
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        if (i + u >= val) break;
-                        const T p00 = g[u][0] * g[u][0], p11 = g[u][1] * g[u][1],
-                                p22 = g[u][2] * g[u][2];
-                        s0 += (double)(p00 * cd[u] + p11 * cm[u] + p22 * cm[u]);
-                        s1 += (double)(g[u][0] * g[u][1] * co[u]);
-                        s2 += (double)(g[u][0] * g[u][2] * co[u]);
-                        s4 += (double)(p00 * cm[u] + p11 * cd[u] + p22 * cm[u]);
-                        s5 += (double)(g[u][1] * g[u][2] * co[u]);
-                        s8 += (double)(p00 * cm[u] + p11 * cm[u] + p22 * cd[u]);
+                for (int u = 0; u < 2; u++) {
+                    if (2 * p + u >= val) break;
+                    const T *g = P.g[u];
+                    if constexpr (ELA) {
+                        const T cd = P.c[u][0], co = P.c[u][1], cm = P.c[u][2];
+                        const T p00 = g[0] * g[0], p11 = g[1] * g[1], p22 = g[2] * g[2];
+                        s0 += (double)(p00 * cd + p11 * cm + p22 * cm);
+                        s1 += (double)(g[0] * g[1] * co);
+                        s2 += (double)(g[0] * g[2] * co);
+                        s4 += (double)(p00 * cm + p11 * cd + p22 * cm);
+                        s5 += (double)(g[1] * g[2] * co);
+                        s8 += (double)(p00 * cm + p11 * cm + p22 * cd);
+                    } else {
+                        const T x = P.c[u][0] * mf::lap_entry(g, g);
+                        s0 += (double)x;
                     }
                 }
-                if (pending) {
-                    load_gathers(hn);
-                    pending = false;
-                }
-                double blk[9] = {s0, s1, s2, s1, s4, s5, s2, s5, s8};
-                if (on) {
-                    store_block9(values + dpos * 9, blk);
-                }
-                const int info = mf::ela_invert_block(blk, (ri.y >> 24) & 7, true);
-                if (on) {
-                    store_block9(prec + (int64_t)lrnode[r] * 9, blk);
-                    if (info != 0 && singular) atomicAdd(singular, 1);
-                }
+            };
+            // LDS reads complete in issue order: each pair's list word is read before the
+            // previous pair's data, so waiting for the word leaves that data in flight
+            Pair A, B;
+            uint32_t wa = word(0), wb = word(1);
+            fetch(wa, A);  // pair 0
+            for (int p = 0; p < np; p += 2) {
+                wa = word(p + 2);
+                fetch(wb, B);  // pair p + 1
+                sum(A, p);
+                if (p + 1 >= np) break;
+                wb = word(p + 3);
+                fetch(wa, A);  // pair p + 2
+                sum(B, p + 1);
+            }
+            if constexpr (ELA) {
+                blk[0] = s0; blk[1] = s1; blk[2] = s2;
+                blk[3] = s1; blk[4] = s4; blk[5] = s5;
+                blk[6] = s2; blk[7] = s5; blk[8] = s8;
             } else {
-                double acc = 0.0;
-                for (int i = 0; i < val; i += 4) {
-                    const uint2 e4 = *reinterpret_cast<const uint2 *>(linc + incOff + i);
-                    const int en[4] = {(int)(e4.x & 0xFFFF), (int)(e4.x >> 16),
-                                       (int)(e4.y & 0xFFFF), (int)(e4.y >> 16)};
-                    T g[4][3], kk[4];
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        const T *sl = stage + (en[u] >> 2) * S;
-                        stage_grad<T, T2>(sl, en[u] & 3, g[u]);
-                        kk[u] = sl[12];
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        if (i + u >= val) break;
-                        const T x = kk[u] * mf::lap_entry(g[u], g[u]);
-                        acc += (double)x;
-                    }
-                }
-                if (pending) {
-                    load_gathers(hn);
-                    pending = false;
-                }
-                if (on) {
-                    values[dpos] = acc;
-                    prec[lrnode[r]] = 1.0 / acc;
-                }
+                blk[0] = s0;
             }
-        }
-
-        __builtin_amdgcn_s_setprio(0);
-
-        // ---- B2: off-diagonal tasks, one lane per block pair: the elements holding both a
-        // and c, two per iteration (independent LDS chains, summed in order); the block
-        // goes to row a, its transpose to row c
-#pragma clang loop unroll(disable)
-        for (int k = 0; k < PB; k++) {
-            const int m = m0 + k * mstep;
-            if (m0 < 0 || m >= nEB) break;
-            const uint32_t inf = tinf[0];
-            const int32_t xp = txp[0];
-#pragma unroll
-            for (int q = 0; q + 1 < PB; q++) {
-                tinf[q] = tinf[q + 1];
-                txp[q] = txp[q + 1];
-            }
-            const bool on = inf != kNoEntry;  // lanes past the tile's last task idle
-            const int ra = inf & 0xFF, cnt = on ? (inf >> 8) & 0xFF : 0, oa = inf >> 16;
-            const int4 ri = on ? lri[ra] : make_int4(0, 0, 0, 0);
-            const uint4 rec = on ? ltask[m * 64 + lane] : make_uint4(0, 0, 0, 0);
-            double acc[NA];
-#pragma unroll
-            for (int q = 0; q < NA; q++) acc[q] = 0.0;
+            __builtin_amdgcn_s_setprio(0);
+        } else if (tOn) {
+            // off-diagonal task: the elements holding both a and c, two per iteration
+            // (independent LDS chains, summed in order); the block goes to row a, its
+            // transpose to row c
+            const uint2 tw = ltw[ti];
+            const uint32_t inf = tw.x;
+            xp = (int32_t)tw.y;
+            const int ra = inf & 0xFF, cnt = (inf >> 8) & 0xFF, oa = inf >> 16;
+            const int4 ri = lri[ra];
+            tEnt = ri.w + oa;
+            const uint4 rc = ltask[ti];
             // one contribution pair (element slot, vertex slots of a and c); a padding
             // entry reads the all-zero slot TE, whose products are +0.0 and leave the
             // sums unchanged
@@ -424,15 +468,15 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                         T bl[9];
                         mf::ela_block(ga[u], gk[u], c0[u], c1[u], c2[u], bl);
 #pragma unroll
-                        for (int q = 0; q < 9; q++) acc[q] += (double)bl[q];
+                        for (int q = 0; q < 9; q++) blk[q] += (double)bl[q];
                     } else {
                         const T x = c0[u] * mf::lap_entry(ga[u], gk[u]);
-                        acc[0] += (double)x;
+                        blk[0] += (double)x;
                     }
                 }
             };
             if (cnt <= mf::kListLen) {
-                const uint32_t w[4] = {rec.x, rec.y, rec.z, rec.w};
+                const uint32_t w[4] = {rc.x, rc.y, rc.z, rc.w};
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     if (2 * u >= cnt) break;
@@ -443,8 +487,8 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                 }
             } else {
                 // more than kListLen elements: walk the mask over row a's incidence list
-                uint64_t msk = (uint64_t)rec.x | (uint64_t)rec.y << 32;
-                const int cl = (int)rec.z, incOff = ri.z & 0xFFFF;
+                uint64_t msk = (uint64_t)rc.x | (uint64_t)rc.y << 32;
+                const int cl = (int)rc.z, incOff = ri.z & 0xFFFF;
                 while (msk) {
                     const int b0 = __builtin_ctzll(msk);
                     msk &= msk - 1;
@@ -464,29 +508,79 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                     add_pair(t, a, kk);
                 }
             }
-            if (pending) {
-                load_gathers(hn);
-                pending = false;
-            }
-            if (on) {
+        }
+        __syncthreads();  // 2: every stage read is done
+
+        // ---- O: the blocks (and in-tile transposes) into the output image
+        if (dOn) {
+            double *o = outv + (int64_t)(dri.w + ((dri.y >> 16) & 0xFF)) * NA;
+#pragma unroll
+            for (int q = 0; q < NA; q++) o[q] = blk[q];
+        } else if (tOn) {
+            double *o = outv + (int64_t)tEnt * NA;
+#pragma unroll
+            for (int q = 0; q < NA; q++) o[q] = blk[q];
+            if (xp <= -2) {
                 if constexpr (ELA) {
-                    store_block9(values + ((int64_t)ri.x + oa) * 9, acc);
-                    if (xp >= 0) {
-                        const double tr[9] = {acc[0], acc[3], acc[6], acc[1], acc[4],
-                                              acc[7], acc[2], acc[5], acc[8]};  // transpose
-                        store_block9(values + (int64_t)xp * 9, tr);
-                    }
+                    const double tr[9] = {blk[0], blk[3], blk[6], blk[1], blk[4],
+                                          blk[7], blk[2], blk[5], blk[8]};  // transpose
+                    double *ot = outv + (int64_t)(-2 - xp) * 9;
+#pragma unroll
+                    for (int q = 0; q < 9; q++) ot[q] = tr[q];
                 } else {
-                    values[(int64_t)ri.x + oa] = acc[0];
-                    if (xp >= 0) values[xp] = acc[0];
+                    outv[-2 - xp] = blk[0];
                 }
             }
         }
-        if (pending) load_gathers(hn);
-        __syncthreads();  // every wave is done with this tile's LDS
-        if (!has_next) break;
-        h = hn;
+        __syncthreads();  // 3
+
+        // ---- W: the diagonal waves invert their blocks into prec; the other compute
+        // waves copy the image to the rows' CSR segments (each segment contiguous in both)
+        if (isDiag) {
+            if (dOn) {
+                const int node = lrnode[r];
+                if constexpr (ELA) {
+                    const int info = mf::ela_invert_block(blk, (dri.y >> 24) & 7, true);
+                    store_block9(prec + (int64_t)node * 9, blk);
+                    if (info != 0 && singular) atomicAdd(singular, 1);
+                } else {
+                    prec[node] = 1.0 / blk[0];
+                }
+            }
+        } else {
+            const int nwr = NC - nDB, wr = wave - nDB;
+            if constexpr (ELA) {
+                // one row per wave: 9 len doubles (~135) as 16-byte pieces, one 8-byte
+                // piece at the end of an odd-length row (8-byte aligned 16-byte stores
+                // are valid on gfx950; the LDS side reads two 8-byte words)
+                for (int rr = wr; rr < R; rr += nwr) {
+                    const int4 ri = lri[rr];
+                    const int n = (ri.y & 0xFFFF) * 9;
+                    const double *src = outv + (int64_t)ri.w * 9;
+                    double *dst = values + (int64_t)ri.x * 9;
+                    for (int j = 2 * lane; j < n; j += 128) {
+                        if (j + 1 < n)
+                            *reinterpret_cast<double2 *>(dst + j) = make_double2(src[j], src[j + 1]);
+                        else
+                            dst[j] = src[j];
+                    }
+                }
+            } else {
+                // four rows per wave, 16 lanes each
+                const int g = lane >> 4, l16 = lane & 15;
+                for (int rr = wr * 4 + g; rr < R; rr += nwr * 4) {
+                    const int4 ri = lri[rr];
+                    const int n = ri.y & 0xFFFF;
+                    for (int j = l16; j < n; j += 16) values[(int64_t)ri.x + j] = outv[ri.w + j];
+                }
+            }
+        }
+        // the next tile's vertices, coordinates and coefficients are written meanwhile
+        __syncthreads();  // 4
+        const int nslot = slot + G;
+        if (!(nslot < per && xcd * per + nslot < ntiles)) break;
         slot = nslot;
+        h = tiles[xcd * per + slot];
     }
 }
 
@@ -570,6 +664,7 @@ int launch_tiled(mf_plan *p, const double *coord, const double *coef, double *va
             G = std::max(1, std::min(per, std::max(1, per_cu) * std::max(1, cus / kXcd)));
         }
         hipLaunchKernelGGL(kern, dim3(G * kXcd), dim3(BT), lds, s, p->tiles, p->ntiles, G,
+                           p->max_node, p->max_elem,
                            p->tile_elems, p->tile_verts, p->tile_nodes, p->tile_rowinfo,
                            reinterpret_cast<const uint4 *>(p->inc), p->task_info, p->task_xpos,
                            p->task_rec, coord, coef, values, prec, singular);
@@ -595,7 +690,7 @@ int launch_variant(mf_plan *p, const double *coord, const double *coef, double *
 }
 
 struct Resolved {
-    int budget, block, max_rows, slot64, slot32, pair_window;
+    int budget, block, max_rows, slot64, slot32;
 };
 
 int resolve_params(int op, const mf_plan_params *params, Resolved *r)
@@ -604,9 +699,7 @@ int resolve_params(int op, const mf_plan_params *params, Resolved *r)
     r->budget = 64 * 1024;
     r->block = 512;
     r->max_rows = op == MF_OP_ELA ? 64 : 128;
-    r->pair_window = kDefaultPairWindow;
     if (params) {
-        if (params->pair_window != 0) r->pair_window = std::max(params->pair_window, 0);
         if (params->tile_lds_bytes > 0) r->budget = params->tile_lds_bytes;
         if (params->block_threads > 0) r->block = params->block_threads;
         if (params->max_tile_rows > 0) r->max_rows = params->max_tile_rows;
@@ -624,16 +717,18 @@ int build_checked(mf::PlanHost &h, int op, int nE, int N, int nnz, const int *e2
                   const Resolved &r, int *lds64, int *lds32)
 {
     int st = mf::build_plan(h, op, nE, N, nnz, e2n, row, col, cb, coord, r.budget, r.max_rows,
-                            r.slot64, r.block, r.pair_window);
+                            r.slot64, r.block);
     if (st != MF_OK) return st;
     st = mf::validate_plan(h, row, col, e2n, r.slot64);
     if (st != MF_OK) return st;
     int m64 = 0, m32 = 0;
     for (const auto &t : h.tiles) {
-        m64 = std::max(m64, mf::tile_lds_layout(t.n_node, t.n_elem, t.n_rows, t.n_inc, t.n_task,
-                                                r.slot64).total);
-        m32 = std::max(m32, mf::tile_lds_layout(t.n_node, t.n_elem, t.n_rows, t.n_inc, t.n_task,
-                                                r.slot32).total);
+        // the front part (fixed offsets) is sized by the largest tile
+        const bool ela = op == MF_OP_ELA;
+        m64 = std::max(m64, mf::tile_lds_layout(ela, h.max_node, h.max_elem, t.n_elem, t.n_rows,
+                                                t.n_inc, t.n_task, t.n_ent, r.slot64).total);
+        m32 = std::max(m32, mf::tile_lds_layout(ela, h.max_node, h.max_elem, t.n_elem, t.n_rows,
+                                                t.n_inc, t.n_task, t.n_ent, r.slot32).total);
     }
     if (m64 + 16 > mf::kMaxLdsBytes) return MF_ERR_ARG;  // one row needs more than a CU has
     *lds64 = m64 + 16;
@@ -696,6 +791,8 @@ extern "C" int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nb
     p->lds_fp64 = lds64;
     p->lds_mixed = lds32;
     p->n_isolated = (int)h.isolated.size();
+    p->max_node = h.max_node;
+    p->max_elem = h.max_elem;
     int rc = upload(&p->tiles, h.tiles);
     if (rc == MF_OK) rc = upload(&p->tile_elems, h.tile_elems);
     if (rc == MF_OK) rc = upload_as(&p->tile_verts, h.tile_verts);

@@ -41,13 +41,14 @@ struct Partitioner {
         if (R > max_rows || R > kMaxTileRows) return false;
         stampId++;
         for (int i = lo; i < hi; i++) nodeStamp[order[i]] = stampId;
-        int64_t TE = 0, NN = R, inc = 0, ent = 0;
+        int64_t TE = 0, NN = R, inc = 0, tasks = 0, ent = 0;
         // tasks: at most one per off-diagonal block, a pair of the tile's rows once
         for (int i = lo; i < hi; i++) {
             const int r = order[i];
+            ent += row[r + 1] - row[r];
             for (int q = row[r]; q < row[r + 1]; q++) {
                 const int c = col[q] - 1;
-                if (c != r && (nodeStamp[c] != stampId || c > r)) ent++;
+                if (c != r && (nodeStamp[c] != stampId || c > r)) tasks++;
             }
         }
         for (int i = lo; i < hi; i++) {
@@ -64,12 +65,12 @@ struct Partitioner {
                 }
             }
         }
-        if (NN > kMaxTileNodes || TE > kMaxTileElems || TE > (int64_t)elems_per_thread(block) * block)
-            return false;
+        if (NN > kMaxTileNodes || TE > max_tile_elems(block)) return false;
         const int64_t inc8 = (inc + 7) / 8 * 8;
-        if (inc8 > kMaxTileInc) return false;
-        if (diag_blocks(ela, R) + (ent + kLanes - 1) / kLanes > kMaxTileBlocks) return false;
-        return tile_lds_layout((int)NN, (int)TE, R, (int)inc8, (int)ent, slot).total <= budget;
+        if (inc8 > kMaxTileInc || ent > kMaxLdsBytes) return false;
+        if (diag_blocks(ela, R) + (tasks + kLanes - 1) / kLanes > max_tile_blocks(block)) return false;
+        return tile_lds_layout(ela, (int)NN, (int)TE, (int)TE, R, (int)inc8, (int)tasks, (int)ent,
+                               slot).total <= budget;
     }
 
     void rcb(int lo, int hi)
@@ -139,7 +140,7 @@ struct Partitioner {
 
 int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, const int *row,
                const int *col, const int *checkBounds, const double *coord, int lds_budget,
-               int max_rows, int slot_bytes, int block, int pair_window)
+               int max_rows, int slot_bytes, int block)
 {
     p = PlanHost{};
     p.op = op;
@@ -149,7 +150,7 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
     p.block = block;
     const bool ela = op == MF_OP_ELA;
     if (nE < 0 || N < 0 || nnz < 0 || max_rows < 1 || max_rows > kMaxTileRows) return MF_ERR_ARG;
-    if (block < kLanes || block % kLanes || kMaxTileBlocks % (block / kLanes)) return MF_ERR_ARG;
+    if (block != 512 && block != 1024) return MF_ERR_ARG;
     if (row[0] != 0 || row[N] != nnz) return MF_ERR_ARG;
     std::vector<int> rowsAll;
     for (int i = 0; i < N; i++) {
@@ -184,9 +185,8 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
     if (coord) part.rcb(0, (int)rowsAll.size());
     else part.bfs_chunks();
 
-    // tile of every row node, and the kernel's dispatch order: tile i runs on XCD i / per
-    // as that XCD's (i % per)-th workgroup
-    const int T = (int)part.leaves.size(), per = (T + kXcds - 1) / kXcds;
+    // tile of every row node (the kernel runs tile i on XCD i / ceil(T / 8))
+    const int T = (int)part.leaves.size();
     std::vector<int> nodeTile((size_t)N, -1);
     for (int i = 0; i < T; i++)
         for (int q = part.leaves[i].first; q < part.leaves[i].second; q++) nodeTile[part.order[q]] = i;
@@ -240,6 +240,11 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
                 p.tile_verts.push_back((uint8_t)nodeLocal[e2n[(int64_t)tel[t] * 4 + k] - 1]);
         }
 
+        // tile entries: the rows' CSR segments in tile row order
+        std::vector<int> ent((size_t)R + 1, 0);
+        for (int ra = 0; ra < R; ra++) ent[ra + 1] = ent[ra] + row[rows[ra] + 1] - row[rows[ra]];
+        h.n_ent = ent[R];
+
         // incidence lists (ascending element = ascending tile element) and entries
         h.inc_start = (int)p.inc.size();
         h.task_start = (int)p.task_info.size();
@@ -255,7 +260,7 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
             if (ela && checkBounds)
                 for (int k = 0; k < 3; k++)
                     if (checkBounds[(int64_t)k * N + a] != 0) mask |= 1 << k;
-            p.tile_rowinfo.push_back(RowInfo{row[a], len | doff << 16 | mask << 24, incOff | val << 16, 0});
+            p.tile_rowinfo.push_back(RowInfo{row[a], len | doff << 16 | mask << 24, incOff | val << 16, ent[ra]});
             for (int j = n2eI[a]; j < n2eI[a + 1]; j++) {
                 const int e = n2eV[j];
                 int va = 0;
@@ -267,21 +272,16 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
                 const int c = col[row[a] + o] - 1;
                 if (c < 0 || c >= N || nodeLocal[c] < 0 || o > 0xFF) return MF_ERR_ARG;
                 if (c == a) continue;
-                // pair (a,c)/(c,a): computed once, by the row with the smaller node id in
-                // one tile, by the earlier tile across tiles
-                const int tj = nodeTile[c];
+                // pair (a,c)/(c,a) inside one tile: computed once, by the row with the
+                // smaller node id, the transpose written into the tile's output image;
+                // a column of another tile: that tile computes (c,a) itself
                 int64_t xpos = -1;
-                if (tj == ti) {
+                if (nodeTile[c] == ti) {
                     if (c < a) continue;
-                    xpos = find_pos(c, a);
-                } else if (pair_window > 0 && tj / per == ti / per && std::abs(tj - ti) <= pair_window) {
-                    if (tj < ti) continue;
-                    xpos = find_pos(c, a);
+                    const int64_t q = find_pos(c, a);
+                    if (q < 0) return MF_ERR_ARG;  // the pattern is not symmetric
+                    xpos = -2 - (ent[nodeLocal[c]] + (q - row[c]));
                 }
-                if (tj != ti && xpos < 0 && pair_window > 0 && tj / per == ti / per &&
-                    std::abs(tj - ti) <= pair_window)
-                    return MF_ERR_ARG;  // the pattern is not symmetric
-                if (tj == ti && xpos < 0) return MF_ERR_ARG;
                 // the elements holding a and c, ascending, with a's and c's slots
                 uint64_t m = 0;
                 uint16_t list[kListLen];
@@ -342,15 +342,13 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
         if (i < 0 || i >= p.N || row[i + 1] != row[i] || rowSeen[i]) return MF_ERR_STATE;
         rowSeen[i] = 1;
     }
-    const int NW = p.block / kLanes;
     std::vector<int> nodeStamp((size_t)p.N, -1), localOf((size_t)p.N, -1), cnt(kMaxTileNodes);
     std::vector<uint64_t> holds(kMaxTileNodes, 0);
     int tileNo = 0;
     for (const TileHdr &h : p.tiles) {
         const int R = h.n_rows;
         if (R < 1 || R > kMaxTileRows || h.n_node < R || h.n_node > kMaxTileNodes ||
-            h.n_elem < 0 || h.n_elem > kMaxTileElems ||
-            h.n_elem > elems_per_thread(p.block) * p.block)
+            h.n_elem < 0 || h.n_elem > max_tile_elems(p.block))
             return MF_ERR_STATE;
         if (h.node_start < 0 || (size_t)h.node_start + h.n_node > p.tile_nodes.size()) return MF_ERR_STATE;
         if (h.elem_start < 0 || (size_t)h.elem_start + h.n_elem > p.tile_elems.size()) return MF_ERR_STATE;
@@ -360,25 +358,23 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
             return MF_ERR_STATE;
         if (h.task_start < 0 || h.n_task < 0 || (size_t)h.task_start + h.n_task > p.task_info.size())
             return MF_ERR_STATE;
+        // one work block per compute wave, and a compute wave without a diagonal block
         const int nDB = diag_blocks(ela, R), nEB = (h.n_task + kLanes - 1) / kLanes;
-        if (nDB + nEB > kMaxTileBlocks || nDB > NW) return MF_ERR_STATE;
-        {  // every block lands in one per-wave slot of the kernel, once
-            std::vector<int> seen(nEB, 0), dseen(nDB, 0);
-            for (int w = 0; w < NW; w++) {
-                int dg, m0, mstep;
-                task_blocks(R, h.n_task, NW, w, dg, m0, mstep);
-                if (dg >= nDB) return MF_ERR_STATE;
-                if (dg >= 0) dseen[dg]++;
-                for (int k = 0; k < kMaxTileBlocks / NW && m0 >= 0; k++)
-                    if (m0 + k * mstep < nEB) seen[m0 + k * mstep]++;
-            }
-            for (int m = 0; m < nEB; m++)
-                if (seen[m] != 1) return MF_ERR_STATE;
-            for (int i = 0; i < nDB; i++)
-                if (dseen[i] != 1) return MF_ERR_STATE;
-        }
-        if (tile_lds_layout(h.n_node, h.n_elem, R, h.n_inc, h.n_task, slot_bytes).total + 16 > kMaxLdsBytes)
+        if (nDB + nEB > max_tile_blocks(p.block) || nDB >= compute_waves(p.block)) return MF_ERR_STATE;
+        if (h.n_ent < R || h.n_node > p.max_node || h.n_elem > p.max_elem ||
+            tile_lds_layout(ela, p.max_node, p.max_elem, h.n_elem, R, h.n_inc, h.n_task, h.n_ent,
+                            slot_bytes).total + 16 > kMaxLdsBytes)
             return MF_ERR_STATE;
+        // row entry offsets: consecutive segments in tile row order
+        {
+            int e = 0;
+            for (int ra = 0; ra < R; ra++) {
+                const RowInfo ri = p.tile_rowinfo[h.rinfo_start + ra];
+                if (ri.ent != e) return MF_ERR_STATE;
+                e += ri.len_diag & 0xFFFF;
+            }
+            if (e != h.n_ent) return MF_ERR_STATE;
+        }
         const int32_t *nodes = p.tile_nodes.data() + h.node_start;
         const uint8_t *verts = p.tile_verts.data() + (size_t)h.elem_start * 4;
         tileNo++;
@@ -463,8 +459,12 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
                     if (m != holds[ec] || (int)rec[2] != ec || rec[3] != 0) return MF_ERR_STATE;
                 }
                 const int64_t xp = p.task_xpos[tk];
-                if (xp >= 0) {
-                    if (xp < row[c] || xp >= row[c + 1] || col[xp] != a + 1 || hit[xp]++)
+                if (xp <= -2) {  // entry (c, a) of this tile's output image
+                    if (ec >= R) return MF_ERR_STATE;
+                    const RowInfo rc = p.tile_rowinfo[h.rinfo_start + ec];
+                    const int64_t j = -2 - xp, oc = j - rc.ent;
+                    if (oc < 0 || oc >= (rc.len_diag & 0xFFFF) || col[row[c] + oc] != a + 1 ||
+                        hit[(size_t)row[c] + oc]++)
                         return MF_ERR_STATE;
                 } else if (xp != -1) {
                     return MF_ERR_STATE;

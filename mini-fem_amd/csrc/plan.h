@@ -1,24 +1,34 @@
 // Tiled gather schedule for one mesh topology (host-built, device-resident).
 //
 // The nodes that own a CSR row are grouped into tiles by recursive coordinate bisection.
-// A tile's workgroup
-//   A) loads the coordinates of every node its elements touch into LDS and computes the
-//      scaled gradients and coefficients of every element incident to one of its rows,
-//      once per tile, into an LDS element stage;
-//   B) sums every CSR block of its rows from the stage, each block in ascending
+// A tile's workgroup has two kinds of waves. Loader waves issue every global load: the
+// next tile's static data and its coordinate and coefficient gathers, held in registers
+// and written into LDS between phases; they never store. Compute waves never load from
+// global memory; they
+//   A) compute the scaled gradients and coefficients of every element incident to one of
+//      the tile's rows, once per tile, into an LDS element stage;
+//   B) sum every CSR block of the tile's rows from the stage, each block in ascending
 //      (color-permuted) element order -- the summation order of the reference's
-//      COLORING build, src/assembly.cc:246-271 -- and writes it once:
+//      COLORING build, src/assembly.cc:246-271:
 //      - a row's incident elements are kept in LDS as an ascending "incidence list";
 //      - an off-diagonal block (row a, column c) is one lane, which reads the elements
 //        holding both a and c, with a's and c's vertex slots in them, from its task
 //        record (or, past 8 elements, walks a mask over row a's incidence list);
 //        block (c,a) is its exact transpose (same elements in the same order, and
 //        IEEE products commute), so a pair is computed once and written twice when
-//        c is a row of the same tile, or of a tile that runs close in time on the same
-//        XCD (the partial cache lines then merge in its L2);
-//      - a diagonal block (the whole incidence list) is one lane, which also inverts
-//        it into the block-Jacobi prec (prec_init + prec_inversion fused).
+//        c is a row of the same tile (a column of another tile: that tile computes
+//        (c,a) itself);
+//      - a diagonal block (the whole incidence list) is one lane;
+//   O) put every block of the tile's rows into an LDS output image of those rows' CSR
+//      segments, laid over the element stage;
+//   W) copy the image row by row to the CSR value array in consecutive 16-byte pieces,
+//      while the diagonal lanes invert their blocks into the block-Jacobi prec
+//      (prec_init + prec_inversion fused). Meanwhile the loader waves write the next
+//      tile's coordinates, coefficients and element vertices into LDS.
 // Every CSR entry and prec block is written exactly once; values need no zero-fill.
+// Keeping loads and stores in different waves matters on gfx950: a wave's vmcnt retires
+// its loads and stores in issue order, so a wave that waited for a load issued after its
+// own stores would wait for those stores too.
 #pragma once
 
 #include <cstdint>
@@ -38,12 +48,24 @@ constexpr int kMaxTileElems = 1024;      // staged elements per tile (the zero s
 constexpr int kMaxTileNodes = 256;       // tile-local node ids are 8-bit
 constexpr int kMaxTileRows = 256;        // tile-local row ids are 8-bit
 constexpr int kMaxTileInc = 4096;        // incidence entries per tile
-constexpr int kMaxTileBlocks = 16;       // work blocks (diagonal + task) per tile
 constexpr int kMaxValence = 64;          // incident elements per row (mask width)
 constexpr int kMaxLdsBytes = 160 * 1024; // LDS of one gfx950 CU
 constexpr int kIncAlign = 4;             // row incidence lists start at multiples of 4
 constexpr int kXcds = 8;                 // MI355X: workgroups are dealt to 8 XCDs round-robin
 constexpr int kListLen = 8;              // contributions held in one task record
+
+// Waves of a workgroup of `block` threads (512 or 1024): loaders, then compute waves.
+MF_HOST_DEVICE constexpr int loader_waves(int block) { return block >= 1024 ? 4 : 2; }
+MF_HOST_DEVICE constexpr int compute_waves(int block) { return block / kLanes - loader_waves(block); }
+// Staged elements per tile: at most two per compute thread.
+MF_HOST_DEVICE constexpr int max_tile_elems(int block)
+{
+    return kMaxTileElems < 2 * kLanes * compute_waves(block) ? kMaxTileElems
+                                                               : 2 * kLanes * compute_waves(block);
+}
+// Work blocks of 64 lanes (diagonal + task) per tile: one per compute wave, and at least
+// one compute wave left for the copy-out while the diagonal waves invert.
+MF_HOST_DEVICE constexpr int max_tile_blocks(int block) { return compute_waves(block); }
 
 // Element-stage slot of one tile element, in units of the stage type (double or float):
 // the (x,y) gradient pairs of vertices 0..3 at 2i (16-byte aligned for double2 reads),
@@ -57,25 +79,37 @@ constexpr int kStageLap = 14;
 
 MF_HOST_DEVICE inline int align16(int x) { return (x + 15) & ~15; }
 
-// LDS of one tile: [element stage: n_elem + 1 slots, the last one all zero] [row info:
-// n_rows x 16 B] [task records: n_task x 16 B] [node coordinates: n_node x 24 B] [row
-// node ids: n_rows x 4 B] [element vertices: n_elem + 1 x 4 B] [incidence lists: n_inc x
-// 2 B, n_inc a multiple of 8].
+// LDS of one tile, in two parts.
+// The front part has the same offsets for every tile of a plan (sized by its largest
+// tile, f_node nodes and f_elem elements): the loader waves write the next tile's data
+// there while the compute waves still read the back part of the current tile.
+//   [node coordinates: f_node x 24 B] [element vertices: f_elem + 1 x 4 B]
+//   [raw coefficients: f_elem x 16 B (ela: lambda, mu) or 8 B (lap: k)]
+// The back part follows the tile's own sizes:
+//   [overlay: the element stage (n_elem + 1 slots, the last one all zero) during A and B,
+//             the output image (n_ent blocks of 9 doubles, ela, or 1, lap, in tile entry
+//             order) during O and W]
+//   [row info: n_rows x 16 B] [task records: n_task x 16 B] [task words: n_task x 8 B]
+//   [row node ids: n_rows x 4 B] [incidence lists: n_inc x 2 B, n_inc a multiple of 8].
 struct TileLds {
-    int stage, rinfo, tasks, coords, rnode, verts, inc, total;
+    int coords, verts, coef, stage, rinfo, tasks, twords, rnode, inc, total;
 };
 
-MF_HOST_DEVICE inline TileLds tile_lds_layout(int n_node, int n_elem, int n_rows, int n_inc,
-                                              int n_task, int slot_bytes)
+MF_HOST_DEVICE inline TileLds tile_lds_layout(bool ela, int f_node, int f_elem, int n_elem,
+                                              int n_rows, int n_inc, int n_task, int n_ent,
+                                              int slot_bytes)
 {
     TileLds L;
-    L.stage = 0;
-    L.rinfo = align16((n_elem + 1) * slot_bytes);
+    L.coords = 0;
+    L.verts = align16(f_node * 24);
+    L.coef = align16(L.verts + (f_elem + 1) * 4);
+    L.stage = L.coef + align16(f_elem * (ela ? 16 : 8));
+    const int stage = (n_elem + 1) * slot_bytes, out = n_ent * (ela ? 72 : 8);
+    L.rinfo = L.stage + align16(stage > out ? stage : out);
     L.tasks = L.rinfo + n_rows * 16;
-    L.coords = L.tasks + n_task * 16;
-    L.rnode = L.coords + n_node * 24;
-    L.verts = L.rnode + n_rows * 4;
-    L.inc = align16(L.verts + (n_elem + 1) * 4);
+    L.twords = L.tasks + n_task * 16;
+    L.rnode = L.twords + n_task * 8;
+    L.inc = align16(L.rnode + n_rows * 4);
     L.total = L.inc + align16(n_inc * 2);
     return L;
 }
@@ -83,19 +117,8 @@ MF_HOST_DEVICE inline TileLds tile_lds_layout(int n_node, int n_elem, int n_rows
 // Diagonal work blocks of a tile: one row per lane.
 MF_HOST_DEVICE inline int diag_blocks(bool, int n_rows) { return (n_rows + kLanes - 1) / kLanes; }
 
-// Work blocks of a tile on `nw` waves: diagonal block i (rows 64i..) on wave i; task block
-// m (tasks 64m..) on the waves without a diagonal block while each holds at most
-// kMaxTileBlocks / nw of them, else on all waves round-robin. A wave's k-th task block is
-// m0 + k * mstep (none when m0 < 0); it holds diagonal block `diag` (-1: none).
-MF_HOST_DEVICE inline void task_blocks(int n_rows, int n_task, int nw, int wave, int &diag,
-                                       int &m0, int &mstep)
-{
-    const int nDB = diag_blocks(true, n_rows), nEB = (n_task + kLanes - 1) / kLanes;
-    const bool spread = nEB > (kMaxTileBlocks / nw) * (nw - nDB);
-    diag = wave < nDB ? wave : -1;
-    m0 = spread ? wave : wave - nDB;
-    mstep = spread ? nw : nw - nDB;
-}
+// Work blocks of a tile on its compute waves: diagonal block i (rows 64i..) on compute
+// wave i, task block m (tasks 64m..) on compute wave diag_blocks + m.
 
 struct TileHdr {
     int32_t elem_start, n_elem;  // into tile_elems / tile_verts (ascending element ids)
@@ -104,7 +127,8 @@ struct TileHdr {
     int32_t rinfo_start;         // into tile_rowinfo: n_rows entries
     int32_t inc_start, n_inc;    // into inc (multiples of 8)
     int32_t task_start, n_task;  // into task_info / task_xpos / task_rec
-    int32_t pad[2];
+    int32_t n_ent;               // CSR entries of the tile's rows
+    int32_t pad;
 };
 
 // Static per-tile-row data (int4 in LDS):
@@ -112,7 +136,8 @@ struct TileHdr {
 //   len_diag: row length | diagonal offset << 16 | Dirichlet mask << 24 (bit k set when
 //             dof k of the node is constrained, ela only);
 //   inc:      tile-local offset of the row's incidence list | valence << 16;
-//   ent:      tile-local index of the row's first entry.
+//   ent:      tile-local index of the row's first entry (rows in tile order, each row's
+//             entries in CSR order).
 struct RowInfo {
     int32_t rowptr;
     int32_t len_diag;
@@ -126,8 +151,8 @@ struct RowInfo {
 // Off-diagonal tasks of a tile, in tile-row order and CSR order inside a row:
 //   task_info[e] = ra | n << 8 | oa << 16  (tile row, number of elements holding both a
 //                  and c, offset of the block in row ra);
-//   task_xpos[e]:  CSR position of block (c,a), written with the transpose, or -1 when
-//                  c's own tile computes it;
+//   task_xpos[e]:  where the transpose (c,a) goes: -2 - j for tile entry j (c a row of
+//                  the same tile), or -1 when c's own tile computes it;
 //   task_rec[e]:   16 bytes. n <= kListLen: 8 contributions t << 4 | a << 2 | k (tile
 //                  element, slots of a and c in it) ascending, the unused ones t = n_elem
 //                  (the zero slot). n > kListLen: {mask low, mask high, c, 0}, bit b of
@@ -152,11 +177,9 @@ struct PlanHost {
 // LDS bytes of one element-stage slot of the widest variant that will run it; tiles are
 // grown while their LDS layout stays <= lds_budget and the kernel's per-thread limits
 // for a workgroup of `block` threads hold. checkBounds (3N, ela) may be null.
-// pair_window: cross-tile pairing between tiles on the same XCD at most this many
-// dispatch slots apart (<= 0: pairs within a tile only).
 int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, const int *row,
                const int *col, const int *checkBounds, const double *coord, int lds_budget,
-               int max_rows, int slot_bytes, int block, int pair_window);
+               int max_rows, int slot_bytes, int block);
 
 // Checks every index the kernel will follow against the array it indexes, that every
 // CSR entry is written exactly once (diagonal, task or transpose) and that every task's
@@ -165,8 +188,5 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
 int validate_plan(const PlanHost &p, const int *row, const int *col, const int *e2n,
                   int slot_bytes);
 
-// Per-thread register limits of the kernel for a workgroup of `block` threads: one
-// staged element per thread.
-inline int elems_per_thread(int) { return 1; }
 
 }  // namespace mf

@@ -95,7 +95,7 @@ class Batch:
 
 
 BATCH_CANDIDATES = [
-    dict(tile_lds_bytes=lds, block_threads=bt, max_tile_rows=254, pair_window=1 << 30)
+    dict(tile_lds_bytes=lds, block_threads=bt, max_tile_rows=254)
     for lds, bt in ((40000, 512), (64000, 512), (80000, 512), (150000, 1024))
 ]
 

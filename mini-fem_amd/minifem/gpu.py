@@ -327,13 +327,12 @@ class Plan:
     (mf_plan_create). assemble() runs one fused iteration: values + inverted prec."""
 
     def __init__(self, mesh, tile_lds_bytes=0, block_threads=0, max_tile_rows=0,
-                 use_coord=True, pair_window=0):
+                 use_coord=True):
         from ._lib import PlanParams
         _ensure()
         self._lib = lib()
         self.mesh = mesh
-        params = PlanParams(int(tile_lds_bytes), int(block_threads), int(max_tile_rows),
-                            int(pair_window))
+        params = PlanParams(int(tile_lds_bytes), int(block_threads), int(max_tile_rows))
         p = ctypes.c_void_p()
         check(self._lib.mf_plan_create(
             ctypes.byref(p), mesh.operatorID, mesh.nbElem, mesh.nbNodes, mesh.nnz,

@@ -218,13 +218,11 @@ class Mesh:
         return cls(coord, e2n, code, nnz, operator_id, coef=coef, colored=colored)
 
 
-def plan_host_stats(mesh, tile_lds_bytes=0, block_threads=0, max_tile_rows=0, use_coord=True,
-                    pair_window=0):
+def plan_host_stats(mesh, tile_lds_bytes=0, block_threads=0, max_tile_rows=0, use_coord=True):
     """Builds and validates the tiled gather schedule on the host (mf_plan_host_stats)."""
     import ctypes
     from ._lib import PlanParams
-    params = PlanParams(int(tile_lds_bytes), int(block_threads), int(max_tile_rows),
-                            int(pair_window))
+    params = PlanParams(int(tile_lds_bytes), int(block_threads), int(max_tile_rows))
     out = np.zeros(8, np.int64)
     check(lib().mf_plan_host_stats(mesh.operatorID, mesh.nbElem, mesh.nbNodes, mesh.nnz,
                                    _ptr(mesh.elemToNode), _ptr(mesh.row), _ptr(mesh.col),

@@ -14,10 +14,9 @@ from . import gpu
 TUNED_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 
 DEFAULT_CANDIDATES = [
-    dict(tile_lds_bytes=lds, block_threads=bt, max_tile_rows=rows)
-    for lds, rows in ((24 * 1024, 64), (36 * 1024, 64), (48 * 1024, 96), (64 * 1024, 128),
-                      (78 * 1024, 128), (100 * 1024, 192), (150 * 1024, 254))
-    for bt in (256, 512)
+    dict(tile_lds_bytes=lds, block_threads=bt, max_tile_rows=254)
+    for lds, bt in ((48 * 1024, 512), (64 * 1024, 512), (78 * 1024, 512), (100 * 1024, 1024),
+                    (120 * 1024, 1024), (150 * 1024, 1024), (155000, 1024))
 ]
 
 

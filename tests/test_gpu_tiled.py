@@ -49,8 +49,8 @@ def test_tiled_bit_exact_30_cube(mf, orc, gpu, op, coef_seed):
     dict(block_threads=512), dict(block_threads=1024),
     dict(tile_lds_bytes=8192),
     dict(tile_lds_bytes=150 * 1024, max_tile_rows=254), dict(use_coord=False),
-    dict(max_tile_rows=1), dict(pair_window=2), dict(pair_window=10 ** 6),
-    dict(pair_window=-1), dict(tile_lds_bytes=8192, pair_window=3),
+    dict(max_tile_rows=1), dict(tile_lds_bytes=8192, block_threads=1024),
+    dict(tile_lds_bytes=150 * 1024, block_threads=1024, max_tile_rows=254),
 ])
 def test_tiled_schedule_variants(mf, orc, gpu, op, params):
     print(f"[tiled variant] op={op} {params}", flush=True)

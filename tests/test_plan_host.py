@@ -12,8 +12,7 @@ from minifem.setup import plan_host_stats
 @pytest.mark.parametrize("params", [
     dict(), dict(tile_lds_bytes=8192), dict(tile_lds_bytes=150 * 1024, max_tile_rows=254),
     dict(max_tile_rows=1), dict(use_coord=False), dict(block_threads=512),
-    dict(block_threads=1024), dict(pair_window=4), dict(pair_window=10 ** 6),
-    dict(pair_window=-1),
+    dict(block_threads=1024),
 ])
 def test_plan_builds_and_validates(mf, n, shuffle, op, params):
     coord, e2n, code, nnz = mf.generate_cube(n, 0.2, 3, shuffle)
@@ -27,10 +26,6 @@ def test_plan_builds_and_validates(mf, n, shuffle, op, params):
     assert (off + 1) // 2 <= s["tasks"] <= off
     if s["tiles"] == 1:
         assert s["tasks"] == off // 2  # every pair inside the one tile: computed once
-    if params.get("pair_window", 0) > 0:
-        # pairing across tiles only removes tasks
-        alone = plan_host_stats(mesh, **dict(params, pair_window=-1))
-        assert s["tasks"] <= alone["tasks"]
     # every (row, incident element) pair once; each row's list padded to 4, each tile's
     # to 8
     assert 4 * mesh.nbElem <= s["incidences"] <= 4 * mesh.nbElem + 3 * mesh.nbNodes + 4 * s["tiles"]
