@@ -61,7 +61,7 @@ __device__ __forceinline__ void store_block9(double *dst, const double (&b)[9])
     dst[8] = b[8];
 }
 
-// One tile element's gradient of vertex v from its stage slot: {x, y, z}.
+// lap stage slot: vertex v's gradient {x, y, z}.
 template <typename T, typename T2>
 __device__ __forceinline__ void stage_grad(const T *sl, int v, T (&g)[3])
 {
@@ -71,6 +71,33 @@ __device__ __forceinline__ void stage_grad(const T *sl, int v, T (&g)[3])
     g[2] = sl[8 + v];
 }
 
+// ela stage slot: vertex v's group {gx, gy, gz, lambda+2mu} at sl + 4v (16-byte aligned).
+__device__ __forceinline__ void ela_vertex(const double *p, double (&g)[3], double &cd)
+{
+    const double2 a = *reinterpret_cast<const double2 *>(p), b = *reinterpret_cast<const double2 *>(p + 2);
+    g[0] = a.x;
+    g[1] = a.y;
+    g[2] = b.x;
+    cd = b.y;
+}
+__device__ __forceinline__ void ela_vertex(const float *p, float (&g)[3], float &cd)
+{
+    const float4 a = *reinterpret_cast<const float4 *>(p);
+    g[0] = a.x;
+    g[1] = a.y;
+    g[2] = a.z;
+    cd = a.w;
+}
+__device__ __forceinline__ void ela_vertex_store(double *p, const double (&g)[3], double cd)
+{
+    *reinterpret_cast<double2 *>(p) = make_double2(g[0], g[1]);
+    *reinterpret_cast<double2 *>(p + 2) = make_double2(g[2], cd);
+}
+__device__ __forceinline__ void ela_vertex_store(float *p, const double (&g)[3], double cd)
+{
+    *reinterpret_cast<float4 *>(p) = make_float4((float)g[0], (float)g[1], (float)g[2], (float)cd);
+}
+
 // Persistent workgroups, XCD-aware. The tiles are cut into 8 contiguous ranges in
 // partition order, range x for XCD x (workgroup b runs on XCD b % 8); the G workgroups of
 // an XCD walk its range together, slot s0, s0 + G, ..., so tiles that share elements and
@@ -78,7 +105,7 @@ __device__ __forceinline__ void stage_grad(const T *sl, int v, T (&g)[3])
 //
 // Waves 0..NC-1 compute, waves NC.. load. Per tile, separated by four barriers:
 //   A  compute: element stage          | load: row data, tasks, incidences into LDS
-//   B  compute: blocks into registers  | load: next tile's static loads, then its gathers
+//   B  compute: blocks into registers  | load: next tile's static loads and gathers
 //   O  compute: blocks into the output image (laid over the stage)
 //   W  compute: image -> values, prec  | load: wait gathers, next vertices/coords/coefs
 // Compute waves issue no global loads and loader waves no stores, so no wave ever waits
@@ -97,7 +124,7 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
     using T = typename std::conditional<PREC == MF_PREC_FP64, double, float>::type;
     using T2 = typename Vec2<T>::type;
     constexpr bool ELA = OP == MF_OP_ELA;
-    constexpr int S = ELA ? mf::kStageEla : mf::kStageLap;
+    constexpr int S = mf::stage_len(ELA, PREC == MF_PREC_FP64);
     constexpr int NA = ELA ? 9 : 1;
     constexpr int NC = mf::compute_waves(BT), CT = NC * 64, LT = BT - CT;
     constexpr int MAXE = mf::max_tile_elems(BT);
@@ -123,11 +150,14 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
     if (wave >= NC) {
         // ------------------------------------------------------------------ loader waves
         const int lt = tid - CT;
-        int nodeId[NPT];
+        // the node and element ids of a tile are loaded one tile earlier than the rest of
+        // its data, so its coordinate and coefficient gathers go out together with its
+        // static loads: one memory round trip per tile
+        int nodeId[NPT], nodeIdN[NPT];
+        int elemId[EPT], elemIdN[EPT];
         double cx[NPT][3];
         // 16-byte records as four scalar arrays each (an array of vectors stays in scratch)
         int rix[NPT], riy[NPT], riz[NPT], riw[NPT];
-        int elemId[EPT];
         uint32_t vert[EPT];
         double k0[EPT], k1[EPT];  // raw coefficients: ela (lambda, mu), lap (k, unused)
         uint32_t inx[IPT], iny[IPT], inz[IPT], inw[IPT];
@@ -140,11 +170,16 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
         auto at = [](int start, int n, int i) -> int64_t {
             return n > 0 ? (int64_t)start + (i < n ? i : n - 1) : 0;
         };
+        auto load_ids = [&](const mf::TileHdr &hh, int (&nid)[NPT], int (&eid)[EPT]) {
+#pragma unroll
+            for (int r = 0; r < NPT; r++) nid[r] = tile_nodes[at(hh.node_start, hh.n_node, lt + r * LT)];
+#pragma unroll
+            for (int r = 0; r < EPT; r++) eid[r] = tile_elems[at(hh.elem_start, hh.n_elem, lt + r * LT)];
+        };
         auto load_static = [&](const mf::TileHdr &hh) {
 #pragma unroll
             for (int r = 0; r < NPT; r++) {
                 const int i = lt + r * LT;
-                nodeId[r] = tile_nodes[at(hh.node_start, hh.n_node, i)];
                 const int4 v = rowinfo[at(hh.rinfo_start, hh.n_rows, i)];
                 rix[r] = v.x;
                 riy[r] = v.y;
@@ -154,7 +189,6 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
             for (int r = 0; r < EPT; r++) {
                 const int i = lt + r * LT;
-                elemId[r] = tile_elems[at(hh.elem_start, hh.n_elem, i)];
                 vert[r] = tile_verts[at(hh.elem_start, hh.n_elem, i)];
             }
 #pragma unroll
@@ -258,22 +292,32 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
             }
         };
 
+        auto valid = [&](int sl) { return sl < per && xcd * per + sl < ntiles; };
+        load_ids(h, nodeId, elemId);
         load_static(h);
         load_gathers();
+        if (valid(slot + G)) load_ids(tiles[xcd * per + slot + G], nodeIdN, elemIdN);
         fill1(h);
         __syncthreads();  // P
         for (;;) {
             // A
             fill2(h);
-            __syncthreads();  // 1
-            // B
             const int nslot = slot + G;
-            const bool has_next = nslot < per && xcd * per + nslot < ntiles;
+            const bool has_next = valid(nslot);
+            if (has_next) {  // the next tile's ids, loaded a tile ago
+#pragma unroll
+                for (int r = 0; r < NPT; r++) nodeId[r] = nodeIdN[r];
+#pragma unroll
+                for (int r = 0; r < EPT; r++) elemId[r] = elemIdN[r];
+            }
+            __syncthreads();  // 1
+            // B: the next tile's static data and gathers, and the ids of the one after
             mf::TileHdr hn = h;
             if (has_next) {
                 hn = tiles[xcd * per + nslot];
                 load_static(hn);
                 load_gathers();
+                if (valid(nslot + G)) load_ids(tiles[xcd * per + nslot + G], nodeIdN, elemIdN);
             }
             __syncthreads();  // 2
             // O
@@ -319,18 +363,19 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                 for (int d = 0; d < 3; d++) c[i][d] = lc[vi[i] * 3 + d];
             mf::elem_grad(c, g);
             T *sl = stage + t * S;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                *reinterpret_cast<T2 *>(sl + 2 * i) = T2{(T)g[i][0], (T)g[i][1]};
-                sl[8 + i] = (T)g[i][2];
-            }
             if constexpr (ELA) {
                 // src/assembly.cc:386-409 with per-element (lambda, mu)
                 const double2 lm = reinterpret_cast<const double2 *>(lds_raw + L.coef)[t];
                 const double cd = lm.x + 2.0 * lm.y, co = lm.x + lm.y, cm = lm.y;
-                *reinterpret_cast<T2 *>(sl + 12) = T2{(T)cd, (T)co};
-                *reinterpret_cast<T2 *>(sl + 14) = T2{(T)cm, (T)0};
+#pragma unroll
+                for (int i = 0; i < 4; i++) ela_vertex_store(sl + 4 * i, g[i], cd);
+                *reinterpret_cast<T2 *>(sl + 16) = T2{(T)co, (T)cm};
             } else {
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    *reinterpret_cast<T2 *>(sl + 2 * i) = T2{(T)g[i][0], (T)g[i][1]};
+                    sl[8 + i] = (T)g[i][2];
+                }
                 const double kk = reinterpret_cast<const double *>(lds_raw + L.coef)[t];
                 *reinterpret_cast<T2 *>(sl + 12) = T2{(T)kk, (T)0};
             }
@@ -376,13 +421,13 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                 for (int u = 0; u < 2; u++) {
                     const int e = u ? (int)(w >> 16) : (int)(w & 0xFFFF);
                     const T *sl = stage + (e >> 2) * S;
-                    stage_grad<T, T2>(sl, e & 3, P.g[u]);
                     if constexpr (ELA) {
-                        const T2 c01 = *reinterpret_cast<const T2 *>(sl + 12);
-                        P.c[u][0] = c01.x;
-                        P.c[u][1] = c01.y;
-                        P.c[u][2] = sl[14];
+                        ela_vertex(sl + 4 * (e & 3), P.g[u], P.c[u][0]);
+                        const T2 c12 = *reinterpret_cast<const T2 *>(sl + 16);
+                        P.c[u][1] = c12.x;
+                        P.c[u][2] = c12.y;
                     } else {
+                        stage_grad<T, T2>(sl, e & 3, P.g[u]);
                         P.c[u][0] = sl[12];
                     }
                 }
@@ -451,14 +496,16 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
                 for (int u = 0; u < 2; u++) {
                     const T *sl = stage + t[u] * S;
-                    stage_grad<T, T2>(sl, a[u], ga[u]);
-                    stage_grad<T, T2>(sl, kk[u], gk[u]);
                     if constexpr (ELA) {
-                        const T2 c01 = *reinterpret_cast<const T2 *>(sl + 12);
-                        c0[u] = c01.x;
-                        c1[u] = c01.y;
-                        c2[u] = sl[14];
+                        T unused;
+                        ela_vertex(sl + 4 * a[u], ga[u], c0[u]);
+                        ela_vertex(sl + 4 * kk[u], gk[u], unused);
+                        const T2 c12 = *reinterpret_cast<const T2 *>(sl + 16);
+                        c1[u] = c12.x;
+                        c2[u] = c12.y;
                     } else {
+                        stage_grad<T, T2>(sl, a[u], ga[u]);
+                        stage_grad<T, T2>(sl, kk[u], gk[u]);
                         c0[u] = sl[12];
                     }
                 }
@@ -695,7 +742,7 @@ struct Resolved {
 
 int resolve_params(int op, const mf_plan_params *params, Resolved *r)
 {
-    const int S = op == MF_OP_ELA ? mf::kStageEla : mf::kStageLap;
+    const bool ela = op == MF_OP_ELA;
     r->budget = 64 * 1024;
     r->block = 512;
     r->max_rows = op == MF_OP_ELA ? 64 : 128;
@@ -704,8 +751,8 @@ int resolve_params(int op, const mf_plan_params *params, Resolved *r)
         if (params->block_threads > 0) r->block = params->block_threads;
         if (params->max_tile_rows > 0) r->max_rows = params->max_tile_rows;
     }
-    r->slot64 = S * 8;
-    r->slot32 = S * 4;
+    r->slot64 = mf::stage_len(ela, true) * 8;
+    r->slot32 = mf::stage_len(ela, false) * 4;
     if (r->block != 512 && r->block != 1024) return MF_ERR_ARG;
     if (r->budget < 4096 || r->budget > mf::kMaxLdsBytes || r->max_rows > mf::kMaxTileRows)
         return MF_ERR_ARG;

@@ -67,15 +67,22 @@ MF_HOST_DEVICE constexpr int max_tile_elems(int block)
 // one compute wave left for the copy-out while the diagonal waves invert.
 MF_HOST_DEVICE constexpr int max_tile_blocks(int block) { return compute_waves(block); }
 
-// Element-stage slot of one tile element, in units of the stage type (double or float):
-// the (x,y) gradient pairs of vertices 0..3 at 2i (16-byte aligned for double2 reads),
-// the z components at 8+i, then the coefficients at 12.. (ela: lambda+2mu, lambda+mu,
-// mu; lap: k). The slot length is an odd number of pairs (an odd multiple of 16 bytes
-// in double, of 8 bytes in float): consecutive elements then start in different LDS
-// bank groups, so the per-element stage writes are conflict-free and the gathers of
-// scattered elements spread over all banks.
-constexpr int kStageEla = 18;
+// Element-stage slot of one tile element, in units of the stage type (double or float).
+//   ela: vertex v's {gx, gy, gz, lambda+2mu} at 4v, then {lambda+mu, mu} at 16: a vertex's
+//        gradient with the first coefficient is two 16-byte LDS reads in double, one in
+//        float. 144 bytes in double; 80 in float (16-byte multiples keep those reads
+//        aligned).
+//   lap: the (x,y) gradient pairs at 2v, the z components at 8+v, k at 12: 112 bytes in
+//        double, 56 in float.
+// Every length is an odd multiple of 16 (double) or 8 (float) bytes: consecutive elements
+// start in different LDS bank groups, so the per-element stage writes are conflict-free
+// and the gathers of scattered elements spread over all banks.
+constexpr int kStageEla64 = 18, kStageEla32 = 20;
 constexpr int kStageLap = 14;
+MF_HOST_DEVICE constexpr int stage_len(bool ela, bool fp64)
+{
+    return ela ? (fp64 ? kStageEla64 : kStageEla32) : kStageLap;
+}
 
 MF_HOST_DEVICE inline int align16(int x) { return (x + 15) & ~15; }
 
