@@ -61,6 +61,9 @@ def main():
                     help="instead: Poisson request stream on 16 resident meshes (1k-1M nodes)")
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--rate", type=float, default=400.0, help="served arrivals per second")
+    ap.add_argument("--write-probe", action="store_true",
+                    help="instead of the FEM step, time the step's output writes alone: the "
+                         "tile rows in the kernel's order vs the same bytes front to back")
     ap.add_argument("--copy", type=int, default=0,
                     help="instead of the FEM step, time the plain HBM copy kernel on this many "
                          "bytes (bandwidth reference and byte-counter calibration)")
@@ -116,6 +119,27 @@ def main():
                         max_tile_rows=args.max_rows)
         log(f"plan: {plan.stats()} ({time.time() - t1:.1f}s)")
     stream = gpu.Stream()
+
+    if args.write_probe:
+        out_bytes = 8 * mesh.nnz * (9 if op == minifem.OP_ELA else 1)
+        res = {}
+        for mode, name in ((0, "tile_rows"), (1, "front_to_back"), (2, "tile_rows_whole_lines"),
+                           (3, "tile_rows_partial_lines")):
+            for _ in range(args.warmup):
+                plan.write_probe(dm.values, mode, stream)
+            e0, e1 = gpu.Event(), gpu.Event()
+            e0.record(stream)
+            for _ in range(args.steps):
+                plan.write_probe(dm.values, mode, stream)
+            e1.record(stream)
+            us = e0.elapsed_ms(e1) / args.steps * 1e3
+            res[name] = {"us": round(us, 1)}
+            if mode < 2:  # modes 2 and 3 write a part of the bytes each
+                res[name]["TB_per_s"] = round(out_bytes / us / 1e6, 3)
+        print(json.dumps({"write_probe": res, "bytes": out_bytes,
+                          "workload": f"{args.op} {args.n}^3 node-shuffled, tiled plan "
+                                      f"{plan.stats()['tiles']} tiles"}), flush=True)
+        return
 
     def step():
         if plan is not None:

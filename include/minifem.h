@@ -288,6 +288,14 @@ int mf_plan_assemble(mf_plan *plan, const double *coord, const double *coef,
                      double *values, double *prec, int precision, int *singular,
                      void *stream);
 
+/* Measurement only, no counterpart in the reference: writes zeros over values[nnz*d]
+ * with the assembly's output stream alone, to time what HBM accepts for it. mode 0:
+ * every tile row's CSR segment in the order mf_plan_assemble writes them; mode 1: the
+ * whole array front to back (the same bytes); mode 2: only the 128-byte lines each
+ * segment covers whole; mode 3: only the partial lines at the segments' ends. The values
+ * must be assembled again afterwards. */
+int mf_plan_write_probe(const mf_plan *plan, double *values, int mode, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -653,6 +653,53 @@ __global__ void k_prec_isolated(const int32_t *__restrict__ nodes, int n,
     for (int q = 0; q < 9; q++) prec[(int64_t)node * 9 + q] = blk[q];
 }
 
+// Write-only probe of the assembly's output stream. Mode 0: every tile row's CSR segment,
+// tiles dealt to XCDs and walked as k_tiled walks them, each segment written front to
+// back in 16-byte pieces (the copy-out of phase W); mode 1: the whole value array front to
+// back (the same bytes: the difference is what the row order costs in HBM); modes 2 and 3:
+// mode 0 split into the 128-byte lines a segment covers whole (2) and the partial lines
+// at its two ends, which it shares with the neighbouring rows (3).
+template <int D>
+__global__ void __launch_bounds__(1024)
+    k_write_probe(const mf::TileHdr *__restrict__ tiles, int ntiles, int G,
+                  const int4 *__restrict__ rowinfo, double *__restrict__ values, int mode,
+                  int64_t total)
+{
+    if (mode == 1) {
+        const int64_t n2 = total / 2, stride = (int64_t)gridDim.x * blockDim.x;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride)
+            reinterpret_cast<double2 *>(values)[i] = make_double2(0.0, 0.0);
+        if (blockIdx.x == 0 && threadIdx.x == 0 && (total & 1)) values[total - 1] = 0.0;
+        return;
+    }
+    const int per = (ntiles + kXcd - 1) / kXcd, xcd = blockIdx.x % kXcd;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int slot = blockIdx.x / kXcd; slot < per && xcd * per + slot < ntiles; slot += G) {
+        const mf::TileHdr h = tiles[xcd * per + slot];
+        for (int r = wave; r < h.n_rows; r += nw) {
+            const int4 ri = rowinfo[h.rinfo_start + r];
+            const int64_t d0 = (int64_t)ri.x * D, d1 = d0 + (int64_t)(ri.y & 0xFFFF) * D;
+            if (mode == 0) {
+                for (int64_t j = d0 + 2 * lane; j < d1; j += 128) {
+                    if (j + 1 < d1) *reinterpret_cast<double2 *>(values + j) = make_double2(0.0, 0.0);
+                    else values[j] = 0.0;
+                }
+                continue;
+            }
+            // doubles [l0, l1) are the segment's whole 128-byte lines (16 doubles each)
+            const int64_t l0 = (d0 + 15) / 16 * 16, l1 = d1 / 16 * 16;
+            if (mode == 2) {
+                for (int64_t j = l0 + 2 * lane; j < l1; j += 128)
+                    *reinterpret_cast<double2 *>(values + j) = make_double2(0.0, 0.0);
+            } else {
+                const int64_t h1 = l0 < d1 ? l0 : d1, t0 = l1 > h1 ? l1 : h1;
+                for (int64_t j = d0 + lane; j < h1; j += 64) values[j] = 0.0;
+                for (int64_t j = t0 + lane; j < d1; j += 64) values[j] = 0.0;
+            }
+        }
+    }
+}
+
 template <typename T>
 int upload(T **dst, const std::vector<T> &src)
 {
@@ -890,6 +937,28 @@ extern "C" int mf_plan_stats(const mf_plan *plan, int64_t out[8])
 {
     if (!plan || !out) return MF_ERR_ARG;
     std::memcpy(out, plan->stats, sizeof(plan->stats));
+    return MF_OK;
+}
+
+extern "C" int mf_plan_write_probe(const mf_plan *plan, double *values, int mode, void *stream)
+{
+    if (!plan || !values || mode < 0 || mode > 3) return MF_ERR_ARG;
+    if (plan->ntiles == 0) return MF_OK;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return MF_ERR_HIP;
+    const int per = (plan->ntiles + kXcd - 1) / kXcd;
+    const int G = std::max(1, std::min(per, cus / kXcd));  // one workgroup per CU
+    const int64_t total = (int64_t)plan->nnz * plan->dim;
+    if (plan->dim == 9)
+        hipLaunchKernelGGL(k_write_probe<9>, dim3(G * kXcd), dim3(1024), 0, s, plan->tiles,
+                           plan->ntiles, G, plan->tile_rowinfo, values, mode, total);
+    else
+        hipLaunchKernelGGL(k_write_probe<1>, dim3(G * kXcd), dim3(1024), 0, s, plan->tiles,
+                           plan->ntiles, G, plan->tile_rowinfo, values, mode, total);
+    MF_CHECK_LAUNCH();
     return MF_OK;
 }
 

@@ -90,6 +90,7 @@ SIGNATURES = {
     "mf_plan_destroy": (_i, [_p]),
     "mf_plan_stats": (_i, [_p, _p]),
     "mf_plan_assemble": (_i, [_p, _p, _p, _p, _p, _i, _p, _p]),
+    "mf_plan_write_probe": (_i, [_p, _p, _i, _p]),
     "mf_plan_host_stats": (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p]),
 }
 

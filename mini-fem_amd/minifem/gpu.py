@@ -354,6 +354,12 @@ class Plan:
                                          int(precision), _p(singular), _sp(stream)),
               "mf_plan_assemble")
 
+    def write_probe(self, values, mode, stream=None):
+        """Zeros over `values` with the assembly's output stream alone (mode 0: tile rows
+        in the kernel's order; mode 1: front to back). Measurement only."""
+        check(self._lib.mf_plan_write_probe(self.ptr, _p(values), int(mode), _sp(stream)),
+              "mf_plan_write_probe")
+
     def __del__(self):
         if getattr(self, "ptr", None):
             self._lib.mf_plan_destroy(self.ptr)
