@@ -273,6 +273,9 @@ def bench_serve(args):
         out[name] = {k: round(v, 4) for k, v in st.items()}
         log(f"{name}: {st}")
         del srv
+    out["throughput_vs_plain"] = {
+        k: round(out[k]["requests_per_s"] / out["plain_fp64_one_at_a_time"]["requests_per_s"], 3)
+        for k in ("batched_fp64", "batched_mixed")}
     out["value"] = out["batched_fp64"]["p99_ms"]
     print(json.dumps(out), flush=True)
 

@@ -38,6 +38,9 @@ extern "C" {
 #define MF_OP_ELA 1
 #define MF_PREC_FP64 0       /* plain FP64 (the drop-in) */
 #define MF_PREC_MIXED 1      /* FP32 element products, FP64 Jacobian/scatter/inverse */
+#define MF_PREC_MIXED_F32COEF 2 /* MF_PREC_MIXED taking the coefficient field as float
+                                   (k[nE] lap, (lambda, mu)[nE] ela): the precision the
+                                   element products use; mf_plan_assemble only */
 
 const char *mf_status_string(int status);
 int mf_version(void);
@@ -282,8 +285,9 @@ int mf_plan_stats(const mf_plan *plan, int64_t out[8]);
 
 /* One iteration: values (assembly, src/FEM.cc:183) and the inverted block-Jacobi prec
  * (prec_init + prec_inversion, src/FEM.cc:199-232) in one launch. Device: coord[3N],
- * coef (NULL = reference constants), values[nnz*d], prec[N*d], singular (may be NULL;
- * incremented per singular ela block). */
+ * coef (NULL = reference constants; points to float with MF_PREC_MIXED_F32COEF),
+ * values[nnz*d], prec[N*d], singular (may be NULL; incremented per singular ela
+ * block). */
 int mf_plan_assemble(mf_plan *plan, const double *coord, const double *coef,
                      double *values, double *prec, int precision, int *singular,
                      void *stream);

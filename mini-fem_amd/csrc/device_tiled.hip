@@ -20,7 +20,7 @@ struct mf_plan {
     int max_node = 0, max_elem = 0;  // the largest tile: sizes the fixed front of its LDS
     int lds_fp64 = 0, lds_mixed = 0;
     int n_isolated = 0;
-    int grid[2] = {0, 0};  // persistent grid per precision, set on its first launch
+    int grid[3] = {0, 0, 0};  // persistent grid per precision mode, set on its first launch
     int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     mf::TileHdr *tiles = nullptr;
     int32_t *tile_elems = nullptr;
@@ -224,7 +224,16 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
             const bool hc = coef != nullptr;
 #pragma unroll
             for (int r = 0; r < EPT; r++) {
-                if constexpr (ELA) {
+                if constexpr (PREC == MF_PREC_MIXED_F32COEF) {  // a float coefficient field
+                    const float *cf = reinterpret_cast<const float *>(hc ? coef : coord);
+                    if constexpr (ELA) {
+                        const float2 lm = reinterpret_cast<const float2 *>(cf)[hc ? elemId[r] : 0];
+                        k0[r] = lm.x;
+                        k1[r] = lm.y;
+                    } else {
+                        k0[r] = cf[hc ? elemId[r] : 0];
+                    }
+                } else if constexpr (ELA) {
                     const double2 lm = reinterpret_cast<const double2 *>(hc ? coef : coord)[hc ? elemId[r] : 0];
                     k0[r] = lm.x;
                     k1[r] = lm.y;
@@ -739,7 +748,7 @@ int launch_tiled(mf_plan *p, const double *coord, const double *coef, double *va
     const int lds = PREC == MF_PREC_FP64 ? p->lds_fp64 : p->lds_mixed;
     auto kern = k_tiled<OP, PREC, BT>;
     if (p->ntiles > 0) {
-        int &G = p->grid[PREC == MF_PREC_FP64 ? 0 : 1];
+        int &G = p->grid[PREC];
         if (G == 0) {
             // first launch: allow the whole LDS for this kernel (the attribute is per
             // kernel, shared by every plan), then the persistent grid = as many workgroups
@@ -967,13 +976,19 @@ extern "C" int mf_plan_assemble(mf_plan *plan, const double *coord, const double
                                 void *stream)
 {
     if (!plan || !values || !prec || (plan->N > 0 && !coord)) return MF_ERR_ARG;
-    if (precision != MF_PREC_FP64 && precision != MF_PREC_MIXED) return MF_ERR_ARG;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    if (plan->op == MF_OP_LAP)
-        return precision == MF_PREC_FP64
-                   ? launch_variant<MF_OP_LAP, MF_PREC_FP64>(plan, coord, coef, values, prec, singular, s)
-                   : launch_variant<MF_OP_LAP, MF_PREC_MIXED>(plan, coord, coef, values, prec, singular, s);
-    return precision == MF_PREC_FP64
-               ? launch_variant<MF_OP_ELA, MF_PREC_FP64>(plan, coord, coef, values, prec, singular, s)
-               : launch_variant<MF_OP_ELA, MF_PREC_MIXED>(plan, coord, coef, values, prec, singular, s);
+    const bool lap = plan->op == MF_OP_LAP;
+    switch (precision) {
+    case MF_PREC_FP64:
+        return lap ? launch_variant<MF_OP_LAP, MF_PREC_FP64>(plan, coord, coef, values, prec, singular, s)
+                   : launch_variant<MF_OP_ELA, MF_PREC_FP64>(plan, coord, coef, values, prec, singular, s);
+    case MF_PREC_MIXED:
+        return lap ? launch_variant<MF_OP_LAP, MF_PREC_MIXED>(plan, coord, coef, values, prec, singular, s)
+                   : launch_variant<MF_OP_ELA, MF_PREC_MIXED>(plan, coord, coef, values, prec, singular, s);
+    case MF_PREC_MIXED_F32COEF:
+        return lap ? launch_variant<MF_OP_LAP, MF_PREC_MIXED_F32COEF>(plan, coord, coef, values, prec, singular, s)
+                   : launch_variant<MF_OP_ELA, MF_PREC_MIXED_F32COEF>(plan, coord, coef, values, prec, singular, s);
+    default:
+        return MF_ERR_ARG;
+    }
 }

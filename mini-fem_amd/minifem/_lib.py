@@ -23,6 +23,7 @@ OP_LAP = 0
 OP_ELA = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_MIXED_F32COEF = 2  # mixed, with the coefficient field given as float32 (plans only)
 
 _i = ctypes.c_int
 _i64 = ctypes.c_int64

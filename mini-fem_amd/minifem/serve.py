@@ -15,6 +15,10 @@ Requests in flight on different slots overlap on the device. The loop polls comp
 events and frees memory and slots as requests finish, so new arrivals are admitted
 while others still run. Latency is measured from arrival to completion of the request's
 last kernel.
+
+The mixed-precision server takes a request's coefficient field as float32, the precision
+its element products use (coordinates stay float64: their differences form the element
+Jacobian), which halves those uploads.
 """
 import bisect
 import collections
@@ -23,7 +27,7 @@ import time
 import numpy as np
 
 from . import gpu
-from ._lib import OP_ELA, PREC_FP64
+from ._lib import OP_ELA, PREC_FP64, PREC_MIXED, PREC_MIXED_F32COEF
 from .setup import Mesh, generate_coef, generate_cube
 
 _ALIGN = 256
@@ -132,9 +136,8 @@ class Server:
     def _views(self, req, off):
         m = self.meshes[req.mesh]
         base = self.arena_mem.ptr + off
-        nin = req.payload.size
-        inp = gpu.DeviceView(base, nin, np.float64, self.arena_mem)
-        vo = base + (req.payload.nbytes + _ALIGN - 1) // _ALIGN * _ALIGN
+        inp = gpu.DeviceView(base, req.host.size, req.host.dtype, self.arena_mem)
+        vo = base + (req.host.nbytes + _ALIGN - 1) // _ALIGN * _ALIGN
         values = gpu.DeviceView(vo, m.nnz * m.dim, np.float64, self.arena_mem)
         po = vo + (values.nbytes + _ALIGN - 1) // _ALIGN * _ALIGN
         prec = gpu.DeviceView(po, m.nbNodes * m.dim, np.float64, self.arena_mem)
@@ -143,7 +146,7 @@ class Server:
     def _footprint_aligned(self, req):
         m = self.meshes[req.mesh]
         a = lambda n: (n + _ALIGN - 1) // _ALIGN * _ALIGN
-        return a(req.payload.nbytes) + a(8 * m.dim * m.nnz) + a(8 * m.dim * m.nbNodes)
+        return a(req.host.nbytes) + a(8 * m.dim * m.nnz) + a(8 * m.dim * m.nbNodes)
 
     def _launch(self, req, slot, inp, values, prec):
         s = self.slots[slot]
@@ -151,24 +154,29 @@ class Server:
         gpu.copy_h2d_async(inp, req.host, s)
         coord = inp if req.kind == "coord" else dm.coord
         coef = inp if req.kind == "coef" else dm.coef
-        self.plans[req.mesh].assemble(coord, coef, values, prec, precision=self.precision, stream=s)
+        precision = self.precision
+        if req.kind == "coef" and req.host.dtype == np.float32:
+            precision = PREC_MIXED_F32COEF
+        self.plans[req.mesh].assemble(coord, coef, values, prec, precision=precision, stream=s)
         self.events[slot].record(s)
 
     def stage(self, requests):
         """Copies each request's payload into pinned host memory: the receive buffer a
-        network front end would hand over."""
+        network front end would hand over (a coefficient field as float32 for the
+        mixed-precision server)."""
         for r in requests:
-            r.host = gpu.HostArray(r.payload.size, np.float64)
+            f32 = self.precision == PREC_MIXED and r.kind == "coef"
+            r.host = gpu.HostArray(r.payload.size, np.float32 if f32 else np.float64)
             r.host.array[:] = r.payload
 
     def run(self, requests, keep_results=False):
         """Serves `requests` (arrival times relative to the start) and returns the stats;
         with keep_results, each request's (values, prec) copied back after it finishes."""
         for r in requests:
-            if self._footprint_aligned(r) > self.arena.size:
-                raise ValueError(f"request {r.rid} needs more than the HBM budget")
             if r.host is None:
                 raise ValueError("stage() the requests first")
+            if self._footprint_aligned(r) > self.arena.size:
+                raise ValueError(f"request {r.rid} needs more than the HBM budget")
         pending = collections.deque(sorted(requests, key=lambda r: r.t_arrival))
         busy = [None] * len(self.slots)  # (request, arena block, views)
         results = {}

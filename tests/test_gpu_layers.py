@@ -7,7 +7,7 @@ on every replay; one member against the oracle; mixed precision within 1e-6 of F
 Serving: every request's (values, prec) served in the mix -- overlapping requests on
 several streams under an HBM budget that forces admission to wait -- equals the same
 request served alone, bit for bit; one served request against the oracle with that
-request's input.
+request's input; the mixed-precision server within 1e-6 of FP64.
 """
 import numpy as np
 import pytest
@@ -60,6 +60,33 @@ def test_batched_mixed_precision_error(mf, gpu):
     for (v0, p0), (v1, _) in zip(b64.results(), bmx.results()):
         rel = np.linalg.norm(v1 - v0) / np.linalg.norm(v0)
         assert rel <= 1e-6, rel
+
+
+def test_serving_mixed_mix_equals_alone_and_error(mf, orc, gpu):
+    """The mixed-precision server (coefficient fields uploaded as float32): every result
+    the same alone and in the mix, and within 1e-6 of FP64."""
+    from minifem.serve import Server, make_requests, served_meshes
+    print("[serve mixed] mix vs alone", flush=True)
+    meshes = served_meshes(count=4, n_lo=1000, n_hi=30000, seed=1000)
+    reqs = make_requests(meshes, count=24, rate_hz=20000.0, seed=5000)
+    fp = max(8 * m.dim * (m.nnz + m.nbNodes) + 8 * max(m.coord.size, m.coef.size)
+             for m in meshes)
+    ref = Server(meshes, hbm_budget=int(2.2 * fp), slots=1)  # plain FP64
+    ref.stage(reqs)
+    v64 = {r.rid: ref.run_alone(r)[0] for r in reqs}
+    del ref
+    srv = Server(meshes, hbm_budget=int(2.2 * fp), slots=4, precision=mf.PREC_MIXED)
+    srv.stage(reqs)
+    assert any(r.host.dtype == np.float32 for r in reqs)
+    _, mixed = srv.run(reqs, keep_results=True)
+    worst = 0.0
+    for r in reqs:
+        v, p = srv.run_alone(r)
+        assert np.array_equal(v, mixed[r.rid][0]), r.rid
+        assert np.array_equal(p, mixed[r.rid][1]), r.rid
+        worst = max(worst, np.linalg.norm(v - v64[r.rid]) / np.linalg.norm(v64[r.rid]))
+    print(f"served mixed vs fp64, worst relative Frobenius error: {worst:.3e}")
+    assert worst <= 1e-6
 
 
 def test_serving_mix_equals_alone(mf, orc, gpu):

@@ -105,6 +105,33 @@ def test_tiled_mixed(mf, orc, gpu, op):
 
 
 @pytest.mark.parametrize("op", [0, 1])
+def test_tiled_mixed_f32_coefficients(mf, orc, gpu, op):
+    """MF_PREC_MIXED_F32COEF takes the coefficient field as float: bit-exact against the
+    oracle's mixed restatement on the coefficients rounded to float, and within 1e-6 of
+    FP64 on the original ones."""
+    print(f"[tiled mixed f32 coef] op={op}", flush=True)
+    coord, e2n, code, nnz = mf.generate_cube(20, 0.2, 1, 0)
+    coef = mf.generate_coef(op, e2n.size // 4, 55)
+    mesh = mf.Mesh(coord, e2n, code, nnz, op, coef=coef)
+    plan = gpu.Plan(mesh)
+    dm = gpu.DeviceMesh(mesh)
+    c32 = gpu.DeviceArray.from_numpy(mesh.coef.astype(np.float32))
+    for _ in range(2):  # the second run must repeat the first bit for bit
+        dm.values.fill(np.nan)
+        dm.prec.fill(np.nan)
+        plan.assemble(dm.coord, c32, dm.values, dm.prec, precision=mf.PREC_MIXED_F32COEF)
+        vals, prec = dm.values.numpy(), dm.prec.numpy()
+        om32 = orc.OracleMesh(coord, e2n, code, nnz, op,
+                              coef=coef.astype(np.float32).astype(np.float64))
+        assert np.array_equal(vals, om32.assemble(mixed=True))
+        assert np.array_equal(prec, om32.prec(vals)[0])
+    ov = orc.OracleMesh(coord, e2n, code, nnz, op, coef=coef).assemble()
+    rel = np.linalg.norm(vals - ov) / np.linalg.norm(ov)
+    print(f"mixed, float coefficients vs fp64 relative Frobenius error: {rel:.3e}")
+    assert rel <= 1e-6
+
+
+@pytest.mark.parametrize("op", [0, 1])
 def test_tiled_tiny_and_isolated(mf, orc, gpu, op):
     # single tet, two tets, and a mesh with an unused node (empty CSR row)
     cases = [
