@@ -28,8 +28,6 @@ struct mf_plan {
     int32_t *tile_nodes = nullptr;
     int4 *tile_rowinfo = nullptr;
     uint16_t *inc = nullptr;
-    uint32_t *task_info = nullptr;
-    int32_t *task_xpos = nullptr;
     uint4 *task_rec = nullptr;
     int32_t *cb = nullptr;
     int32_t *isolated = nullptr;
@@ -115,8 +113,7 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
     k_tiled(const mf::TileHdr *__restrict__ tiles, int ntiles, int G, int f_node, int f_elem,
             const int32_t *__restrict__ tile_elems, const uint32_t *__restrict__ tile_verts,
             const int32_t *__restrict__ tile_nodes, const int4 *__restrict__ rowinfo,
-            const uint4 *__restrict__ inc8, const uint32_t *__restrict__ task_info,
-            const int32_t *__restrict__ task_xpos, const uint4 *__restrict__ task_rec,
+            const uint4 *__restrict__ inc8, const uint4 *__restrict__ task_rec,
             const double *__restrict__ coord,
             const double *__restrict__ coef, double *__restrict__ values,
             double *__restrict__ prec, int *__restrict__ singular)
@@ -162,8 +159,6 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
         double k0[EPT], k1[EPT];  // raw coefficients: ela (lambda, mu), lap (k, unused)
         uint32_t inx[IPT], iny[IPT], inz[IPT], inw[IPT];
         uint32_t rcx[TPT], rcy[TPT], rcz[TPT], rcw[TPT];
-        uint32_t tinf[TPT];
-        int32_t txp[TPT];
         // every load is unconditional, from an index clamped into the tile's range (the
         // fill skips the lanes past it): no branch between the loads, so they all stay in
         // flight together
@@ -207,8 +202,6 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                 rcy[r] = v.y;
                 rcz[r] = v.z;
                 rcw[r] = v.w;
-                tinf[r] = task_info[e];
-                txp[r] = task_xpos[e];
             }
         };
         auto load_gathers = [&]() {
@@ -296,7 +289,6 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                 const int i = lt + r * LT;
                 if (i < hh.n_task) {
                     reinterpret_cast<uint4 *>(lds_raw + L.tasks)[i] = make_uint4(rcx[r], rcy[r], rcz[r], rcw[r]);
-                    reinterpret_cast<uint2 *>(lds_raw + L.twords)[i] = make_uint2(tinf[r], (uint32_t)txp[r]);
                 }
             }
         };
@@ -351,7 +343,6 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
         double *outv = reinterpret_cast<double *>(lds_raw + L.stage);  // the output image
         const int4 *lri = reinterpret_cast<const int4 *>(lds_raw + L.rinfo);
         const uint4 *ltask = reinterpret_cast<const uint4 *>(lds_raw + L.tasks);
-        const uint2 *ltw = reinterpret_cast<const uint2 *>(lds_raw + L.twords);
         const double *lc = reinterpret_cast<const double *>(lds_raw + L.coords);
         const int *lrnode = reinterpret_cast<const int *>(lds_raw + L.rnode);
         const uint32_t *lverts = reinterpret_cast<const uint32_t *>(lds_raw + L.verts);
@@ -406,7 +397,7 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
         const int ti = m * 64 + lane;
         const bool tOn = !isDiag && m < nEB && ti < h.n_task;
         int tEnt = 0;                            // tile entry of the task's block
-        int32_t xp = -1;
+        int xEnt = -1;                           // tile entry of its transpose, or -1
         if (isDiag) {
             // the diagonal chains are the tile's critical path: this wave's instructions
             // go first on its SIMD while it sums them
@@ -490,13 +481,11 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
             // off-diagonal task: the elements holding both a and c, two per iteration
             // (independent LDS chains, summed in order); the block goes to row a, its
             // transpose to row c
-            const uint2 tw = ltw[ti];
-            const uint32_t inf = tw.x;
-            xp = (int32_t)tw.y;
-            const int ra = inf & 0xFF, cnt = (inf >> 8) & 0xFF, oa = inf >> 16;
+            const uint4 rc = ltask[ti];
+            const int ra = rc.w & 0xFF, oa = (rc.w >> 8) & 0xFF, cl = (rc.w >> 16) & 0xFF;
             const int4 ri = lri[ra];
             tEnt = ri.w + oa;
-            const uint4 rc = ltask[ti];
+            if (cl < R) xEnt = lri[cl].w + (int)(rc.w >> 24);  // c is a row of this tile
             // one contribution pair (element slot, vertex slots of a and c); a padding
             // entry reads the all-zero slot TE, whose products are +0.0 and leave the
             // sums unchanged
@@ -531,20 +520,21 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                     }
                 }
             };
-            if (cnt <= mf::kListLen) {
-                const uint32_t w[4] = {rc.x, rc.y, rc.z, rc.w};
+            if (rc.x != mf::kMaskRecord) {
+                // up to kListLen contributions, padded with the zero slot TE
+                const uint32_t w[3] = {rc.x, rc.y, rc.z};
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    if (2 * u >= cnt) break;
+                for (int u = 0; u < 3; u++) {
                     const int e0 = (int)(w[u] & 0xFFFF), e1 = (int)(w[u] >> 16);
+                    if ((e0 >> 4) == TE) break;
                     const int t[2] = {e0 >> 4, e1 >> 4}, a[2] = {(e0 >> 2) & 3, (e1 >> 2) & 3},
                               kk[2] = {e0 & 3, e1 & 3};
                     add_pair(t, a, kk);
                 }
             } else {
                 // more than kListLen elements: walk the mask over row a's incidence list
-                uint64_t msk = (uint64_t)rc.x | (uint64_t)rc.y << 32;
-                const int cl = (int)rc.z, incOff = ri.z & 0xFFFF;
+                uint64_t msk = (uint64_t)rc.y | (uint64_t)rc.z << 32;
+                const int incOff = ri.z & 0xFFFF;
                 while (msk) {
                     const int b0 = __builtin_ctzll(msk);
                     msk &= msk - 1;
@@ -576,15 +566,15 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
             double *o = outv + (int64_t)tEnt * NA;
 #pragma unroll
             for (int q = 0; q < NA; q++) o[q] = blk[q];
-            if (xp <= -2) {
+            if (xEnt >= 0) {
                 if constexpr (ELA) {
                     const double tr[9] = {blk[0], blk[3], blk[6], blk[1], blk[4],
                                           blk[7], blk[2], blk[5], blk[8]};  // transpose
-                    double *ot = outv + (int64_t)(-2 - xp) * 9;
+                    double *ot = outv + (int64_t)xEnt * 9;
 #pragma unroll
                     for (int q = 0; q < 9; q++) ot[q] = tr[q];
                 } else {
-                    outv[-2 - xp] = blk[0];
+                    outv[xEnt] = blk[0];
                 }
             }
         }
@@ -734,7 +724,7 @@ void free_plan(mf_plan *p)
 {
     if (!p) return;
     void *ptrs[] = {p->tiles, p->tile_elems, p->tile_verts, p->tile_nodes, p->tile_rowinfo,
-                    p->inc,   p->task_info,  p->task_xpos,  p->task_rec,   p->cb,
+                    p->inc,   p->task_rec,   p->cb,
                     p->isolated};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -769,8 +759,8 @@ int launch_tiled(mf_plan *p, const double *coord, const double *coef, double *va
         hipLaunchKernelGGL(kern, dim3(G * kXcd), dim3(BT), lds, s, p->tiles, p->ntiles, G,
                            p->max_node, p->max_elem,
                            p->tile_elems, p->tile_verts, p->tile_nodes, p->tile_rowinfo,
-                           reinterpret_cast<const uint4 *>(p->inc), p->task_info, p->task_xpos,
-                           p->task_rec, coord, coef, values, prec, singular);
+                           reinterpret_cast<const uint4 *>(p->inc), p->task_rec, coord, coef,
+                           values, prec, singular);
         MF_CHECK_LAUNCH();
     }
     if (p->n_isolated > 0) {
@@ -845,13 +835,12 @@ void fill_stats(const mf::PlanHost &h, int lds64, int block, int64_t out[8])
 {
     out[0] = (int64_t)h.tiles.size();
     out[1] = (int64_t)h.tile_elems.size();
-    out[2] = (int64_t)h.task_info.size();
+    out[2] = (int64_t)h.task_rec.size() / 4;
     out[3] = (int64_t)h.inc.size();
     out[4] = (int64_t)h.tiles.size() * sizeof(mf::TileHdr) + (int64_t)h.tile_elems.size() * 4 +
              (int64_t)h.tile_verts.size() + (int64_t)h.tile_nodes.size() * 4 +
              (int64_t)h.tile_rowinfo.size() * sizeof(mf::RowInfo) + (int64_t)h.inc.size() * 2 +
-             (int64_t)h.task_rec.size() * 4 + (int64_t)h.task_info.size() * 4 +
-             (int64_t)h.task_xpos.size() * 4;
+             (int64_t)h.task_rec.size() * 4;
     out[5] = lds64;
     out[6] = block;
     out[7] = h.max_rows;
@@ -903,8 +892,6 @@ extern "C" int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nb
     if (rc == MF_OK) rc = upload_as(&p->tile_rowinfo, h.tile_rowinfo);
     if (rc == MF_OK) rc = upload(&p->inc, h.inc);
     if (rc == MF_OK) rc = upload_as(&p->task_rec, h.task_rec);
-    if (rc == MF_OK) rc = upload(&p->task_info, h.task_info);
-    if (rc == MF_OK) rc = upload(&p->task_xpos, h.task_xpos);
     if (rc == MF_OK) rc = upload(&p->isolated, h.isolated);
     if (rc == MF_OK && checkBounds && operatorID == MF_OP_ELA)
         rc = upload(&p->cb, std::vector<int32_t>(checkBounds, checkBounds + (size_t)nbNodes * 3));

@@ -247,7 +247,7 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
 
         // incidence lists (ascending element = ascending tile element) and entries
         h.inc_start = (int)p.inc.size();
-        h.task_start = (int)p.task_info.size();
+        h.task_start = (int)(p.task_rec.size() / 4);
         int incOff = 0;
         for (int ra = 0; ra < R; ra++) {
             const int a = rows[ra], len = row[a + 1] - row[a];
@@ -275,12 +275,13 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
                 // pair (a,c)/(c,a) inside one tile: computed once, by the row with the
                 // smaller node id, the transpose written into the tile's output image;
                 // a column of another tile: that tile computes (c,a) itself
-                int64_t xpos = -1;
+                int oc = 0;
                 if (nodeTile[c] == ti) {
                     if (c < a) continue;
                     const int64_t q = find_pos(c, a);
                     if (q < 0) return MF_ERR_ARG;  // the pattern is not symmetric
-                    xpos = -2 - (ent[nodeLocal[c]] + (q - row[c]));
+                    oc = (int)(q - row[c]);
+                    if (oc > 0xFF) return MF_ERR_ARG;
                 }
                 // the elements holding a and c, ascending, with a's and c's slots
                 uint64_t m = 0;
@@ -298,33 +299,31 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
                     if (cnt < kListLen) list[cnt] = (uint16_t)(elemLocal[n2eV[j]] << 4 | va << 2 | vc);
                     cnt++;
                 }
-                if (cnt < 1 || cnt > 0xFF) return MF_ERR_ARG;
-                p.task_info.push_back((uint32_t)(ra | cnt << 8 | o << 16));
-                p.task_xpos.push_back((int32_t)xpos);
+                if (cnt < 1) return MF_ERR_ARG;
                 if (cnt <= kListLen) {
                     for (int q = cnt; q < kListLen; q++) list[q] = (uint16_t)(TE << 4);
-                    for (int w = 0; w < 4; w++)
+                    for (int w = 0; w < 3; w++)
                         p.task_rec.push_back((uint32_t)list[2 * w] | (uint32_t)list[2 * w + 1] << 16);
                 } else {
+                    p.task_rec.push_back(kMaskRecord);
                     p.task_rec.push_back((uint32_t)m);
                     p.task_rec.push_back((uint32_t)(m >> 32));
-                    p.task_rec.push_back((uint32_t)nodeLocal[c]);
-                    p.task_rec.push_back(0u);
                 }
+                p.task_rec.push_back((uint32_t)(ra | o << 8 | nodeLocal[c] << 16 | oc << 24));
             }
             while (p.inc.size() % kIncAlign) p.inc.push_back(0);
             incOff = (int)p.inc.size() - h.inc_start;
         }
         while (p.inc.size() % 8) p.inc.push_back(0);
         h.n_inc = (int)p.inc.size() - h.inc_start;
-        h.n_task = (int)p.task_info.size() - h.task_start;
+        h.n_task = (int)(p.task_rec.size() / 4) - h.task_start;
         p.tiles.push_back(h);
         p.max_node = std::max(p.max_node, h.n_node);
         p.max_elem = std::max(p.max_elem, TE);
         p.max_rows = std::max(p.max_rows, R);
         for (int i = h.node_start; i < (int)p.tile_nodes.size(); i++) nodeLocal[p.tile_nodes[i]] = -1;
         for (int e : tel) elemLocal[e] = -1;
-        if (p.inc.size() > (size_t)INT32_MAX - 100000 || p.task_info.size() > (size_t)INT32_MAX / 2)
+        if (p.inc.size() > (size_t)INT32_MAX - 100000 || p.task_rec.size() > (size_t)INT32_MAX / 2)
             return MF_ERR_NOMEM;
     }
     return MF_OK;
@@ -335,8 +334,8 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
 {
     const bool ela = p.op == MF_OP_ELA;
     if (p.tile_verts.size() != p.tile_elems.size() * 4) return MF_ERR_STATE;
-    if (p.task_rec.size() != p.task_info.size() * 4 || p.task_xpos.size() != p.task_info.size())
-        return MF_ERR_STATE;
+    if (p.task_rec.size() % 4) return MF_ERR_STATE;
+    const size_t nTasks = p.task_rec.size() / 4;
     std::vector<uint8_t> hit((size_t)p.nnz, 0), rowSeen((size_t)p.N, 0);
     for (int i : p.isolated) {
         if (i < 0 || i >= p.N || row[i + 1] != row[i] || rowSeen[i]) return MF_ERR_STATE;
@@ -356,7 +355,7 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
         if (h.inc_start < 0 || h.inc_start % 8 || h.n_inc < 0 || h.n_inc % 8 ||
             h.n_inc > kMaxTileInc || (size_t)h.inc_start + h.n_inc > p.inc.size())
             return MF_ERR_STATE;
-        if (h.task_start < 0 || h.n_task < 0 || (size_t)h.task_start + h.n_task > p.task_info.size())
+        if (h.task_start < 0 || h.n_task < 0 || (size_t)h.task_start + h.n_task > nTasks)
             return MF_ERR_STATE;
         // one work block per compute wave, and a compute wave without a diagonal block
         const int nDB = diag_blocks(ela, R), nEB = (h.n_task + kLanes - 1) / kLanes;
@@ -427,15 +426,15 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
             }
             if (cnt[ra] != val) return MF_ERR_STATE;
             // the row's tasks: exact masks, each block and its transpose written once
-            for (; tk < tkEnd && (int)(p.task_info[tk] & 0xFF) == ra; tk++) {
-                const uint32_t inf = p.task_info[tk];
-                const int cnt = (inf >> 8) & 0xFF, oa = inf >> 16;
+            for (; tk < tkEnd && (int)(p.task_rec[(size_t)tk * 4 + 3] & 0xFF) == ra; tk++) {
+                const uint32_t *rec = p.task_rec.data() + (size_t)tk * 4;
+                const int oa = (rec[3] >> 8) & 0xFF, cl = (rec[3] >> 16) & 0xFF, oc = rec[3] >> 24;
                 if (oa >= len) return MF_ERR_STATE;
                 const int c = col[row[a] + oa] - 1;
                 if (c == a || hit[(size_t)row[a] + oa]++) return MF_ERR_STATE;
-                const uint32_t *rec = p.task_rec.data() + (size_t)tk * 4;
                 const int ec = nodeStamp[c] == tileNo ? localOf[c] : -1;
-                if (ec < 0 || __builtin_popcountll(holds[ec]) != cnt || cnt < 1) return MF_ERR_STATE;
+                const int cnt = ec < 0 ? 0 : __builtin_popcountll(holds[ec]);
+                if (ec < 0 || cl != ec || cnt < 1) return MF_ERR_STATE;
                 if (cnt <= kListLen) {
                     // the exact ascending list of (element, slot of a, slot of c), padded
                     // with the zero slot
@@ -455,18 +454,14 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
                         if (((rec[q / 2] >> (16 * (q % 2))) & 0xFFFF) != (uint32_t)(h.n_elem << 4))
                             return MF_ERR_STATE;
                 } else {
-                    const uint64_t m = (uint64_t)rec[0] | (uint64_t)rec[1] << 32;
-                    if (m != holds[ec] || (int)rec[2] != ec || rec[3] != 0) return MF_ERR_STATE;
+                    const uint64_t m = (uint64_t)rec[1] | (uint64_t)rec[2] << 32;
+                    if (rec[0] != kMaskRecord || m != holds[ec]) return MF_ERR_STATE;
                 }
-                const int64_t xp = p.task_xpos[tk];
-                if (xp <= -2) {  // entry (c, a) of this tile's output image
-                    if (ec >= R) return MF_ERR_STATE;
-                    const RowInfo rc = p.tile_rowinfo[h.rinfo_start + ec];
-                    const int64_t j = -2 - xp, oc = j - rc.ent;
-                    if (oc < 0 || oc >= (rc.len_diag & 0xFFFF) || col[row[c] + oc] != a + 1 ||
-                        hit[(size_t)row[c] + oc]++)
+                if (ec < R) {  // c is a row of this tile: the task also writes (c, a)
+                    if (oc >= (p.tile_rowinfo[h.rinfo_start + ec].len_diag & 0xFFFF) ||
+                        col[row[c] + oc] != a + 1 || hit[(size_t)row[c] + oc]++)
                         return MF_ERR_STATE;
-                } else if (xp != -1) {
+                } else if (oc != 0) {
                     return MF_ERR_STATE;
                 }
             }

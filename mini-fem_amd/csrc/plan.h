@@ -52,7 +52,8 @@ constexpr int kMaxValence = 64;          // incident elements per row (mask widt
 constexpr int kMaxLdsBytes = 160 * 1024; // LDS of one gfx950 CU
 constexpr int kIncAlign = 4;             // row incidence lists start at multiples of 4
 constexpr int kXcds = 8;                 // MI355X: workgroups are dealt to 8 XCDs round-robin
-constexpr int kListLen = 8;              // contributions held in one task record
+constexpr int kListLen = 6;              // contributions held in one task record
+constexpr uint32_t kMaskRecord = 0xFFFFFFFFu;  // first word of a mask task record
 
 // Waves of a workgroup of `block` threads (512 or 1024): loaders, then compute waves.
 MF_HOST_DEVICE constexpr int loader_waves(int block) { return block >= 1024 ? 4 : 2; }
@@ -96,10 +97,10 @@ MF_HOST_DEVICE inline int align16(int x) { return (x + 15) & ~15; }
 //   [overlay: the element stage (n_elem + 1 slots, the last one all zero) during A and B,
 //             the output image (n_ent blocks of 9 doubles, ela, or 1, lap, in tile entry
 //             order) during O and W]
-//   [row info: n_rows x 16 B] [task records: n_task x 16 B] [task words: n_task x 8 B]
-//   [row node ids: n_rows x 4 B] [incidence lists: n_inc x 2 B, n_inc a multiple of 8].
+//   [row info: n_rows x 16 B] [task records: n_task x 16 B] [row node ids: n_rows x 4 B]
+//   [incidence lists: n_inc x 2 B, n_inc a multiple of 8].
 struct TileLds {
-    int coords, verts, coef, stage, rinfo, tasks, twords, rnode, inc, total;
+    int coords, verts, coef, stage, rinfo, tasks, rnode, inc, total;
 };
 
 MF_HOST_DEVICE inline TileLds tile_lds_layout(bool ela, int f_node, int f_elem, int n_elem,
@@ -114,8 +115,7 @@ MF_HOST_DEVICE inline TileLds tile_lds_layout(bool ela, int f_node, int f_elem, 
     const int stage = (n_elem + 1) * slot_bytes, out = n_ent * (ela ? 72 : 8);
     L.rinfo = L.stage + align16(stage > out ? stage : out);
     L.tasks = L.rinfo + n_rows * 16;
-    L.twords = L.tasks + n_task * 16;
-    L.rnode = L.twords + n_task * 8;
+    L.rnode = L.tasks + n_task * 16;
     L.inc = align16(L.rnode + n_rows * 4);
     L.total = L.inc + align16(n_inc * 2);
     return L;
@@ -155,15 +155,17 @@ struct RowInfo {
 // Incidence entry: t << 2 | a, t the tile element, a the row node's vertex slot in it.
 // Each row's list starts at a multiple of kIncAlign entries (zero padding between lists:
 // element 0, read but never summed), so a lane reads 4 entries with one 8-byte access.
-// Off-diagonal tasks of a tile, in tile-row order and CSR order inside a row:
-//   task_info[e] = ra | n << 8 | oa << 16  (tile row, number of elements holding both a
-//                  and c, offset of the block in row ra);
-//   task_xpos[e]:  where the transpose (c,a) goes: -2 - j for tile entry j (c a row of
-//                  the same tile), or -1 when c's own tile computes it;
-//   task_rec[e]:   16 bytes. n <= kListLen: 8 contributions t << 4 | a << 2 | k (tile
-//                  element, slots of a and c in it) ascending, the unused ones t = n_elem
-//                  (the zero slot). n > kListLen: {mask low, mask high, c, 0}, bit b of
-//                  the mask set when the b-th element of row ra's incidence list holds c.
+// Off-diagonal tasks of a tile, in tile-row order and CSR order inside a row, one 16-byte
+// record each (task_rec, 4 words), for the block (a,c) of tile row a and column c:
+//   words 0-2: the n elements holding both a and c. n <= kListLen: six u16 contributions
+//              t << 4 | a << 2 | k (tile element, slots of a and c in it) ascending, the
+//              unused ones t = n_elem (the zero slot). n > kListLen: word 0 = kMaskRecord,
+//              words 1-2 a 64-bit mask, bit b set when the b-th element of row a's
+//              incidence list holds c.
+//   word 3:    ra | oa << 8 | cl << 16 | oc << 24: a's tile row, the block's offset in
+//              a's row, c's tile-local node, and (when cl < n_rows: c is a row of the same
+//              tile, and the task also writes the transpose (c,a)) the offset of (c,a) in
+//              c's row. A column of another tile: that tile computes (c,a) itself.
 struct PlanHost {
     int op = 0, nE = 0, N = 0, nnz = 0, block = 512;
     std::vector<TileHdr> tiles;
@@ -172,8 +174,6 @@ struct PlanHost {
     std::vector<int32_t> tile_nodes;   // global node ids (0-based)
     std::vector<RowInfo> tile_rowinfo; // per tile row
     std::vector<uint16_t> inc;
-    std::vector<uint32_t> task_info;
-    std::vector<int32_t> task_xpos;
     std::vector<uint32_t> task_rec;    // 4 words per task
     std::vector<int32_t> isolated;     // nodes with an empty row (no element)
     int max_node = 0, max_elem = 0, max_rows = 0;
