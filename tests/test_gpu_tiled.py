@@ -214,3 +214,27 @@ def test_tiled_large_properties(mf, orc, gpu, op):
     dm.iterate_colored()
     assert np.array_equal(vals, dm.values.numpy())
     assert np.array_equal(prec, dm.prec.numpy())
+
+
+@pytest.mark.parametrize("op", [0, 1])
+def test_write_probe_covers_every_value(mf, orc, gpu, op):
+    """mf_plan_write_probe (measurement only): mode 0 (tile rows), mode 1 (front to back)
+    and modes 2 + 3 (whole lines + partial end lines) each overwrite every value exactly
+    where the assembly writes, and nothing past the array."""
+    print(f"[write probe] op={op}", flush=True)
+    mesh, _ = build(mf, orc, 14, op, seed=5, shuffle=207, oracle=False)
+    plan = gpu.Plan(mesh)
+    n = mesh.nnz * mesh.dim
+    buf = gpu.DeviceArray(n + 64, np.float64)  # a guard band past the values
+    for modes in ((0,), (1,), (2, 3)):
+        buf.fill(np.nan)
+        for m in modes:
+            plan.write_probe(buf, m)
+        out = buf.numpy()
+        assert np.all(out[:n] == 0.0), modes
+        assert np.all(np.isnan(out[n:])), modes
+    if op == 1:  # ela rows (~1 KB) cover whole lines; lap rows (~120 B) rarely do
+        buf.fill(np.nan)
+        plan.write_probe(buf, 2)
+        part = buf.numpy()[:n]
+        assert np.any(np.isnan(part)) and np.any(part == 0.0)
