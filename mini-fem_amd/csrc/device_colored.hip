@@ -3,10 +3,10 @@
 //   prec_init()       -> mf_prec_init
 //   prec_inversion()  -> mf_prec_inversion
 //   compute_double_norm() -> mf_norm
-// The assembly is one launch per color with one element per lane: elements of one
-// color share no node, so their scatter-adds never touch the same CSR entry and each
-// pass is race-free and deterministic; the per-entry summation order is the color
-// order, i.e. the reference COLORING build's element order.
+// The assembly is one launch per color with 16 lanes per element (one per element block):
+// elements of one color share no node, so their scatter-adds never touch the same CSR
+// entry and each pass is race-free and deterministic; the per-entry summation order is
+// the color order, i.e. the reference COLORING build's element order.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -27,39 +27,23 @@ __global__ void __launch_bounds__(kBlock) k_fill(double *__restrict__ p, int64_t
     for (; i < n; i += stride) p[i] = v;
 }
 
-// CSR positions of the 16 (j,k) pairs of one element: elemToEdge when present, else
-// the reference's row search (src/assembly.cc:415-421).
-__device__ __forceinline__ void elem_positions(const int *__restrict__ row,
-                                               const int *__restrict__ col,
-                                               const int *__restrict__ e2e, int e, int4 v,
-                                               int (&pos)[16])
+// CSR position of the (j,k) pair of one element: elemToEdge when present, else the
+// reference's row search (src/assembly.cc:415-421).
+__device__ __forceinline__ int elem_position(const int *__restrict__ row,
+                                             const int *__restrict__ col,
+                                             const int *__restrict__ e2e, int e, int nj, int nk,
+                                             int q)
 {
-    if (e2e) {
-        const int4 *q = reinterpret_cast<const int4 *>(e2e + (int64_t)e * 16);
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int4 t = q[r];
-            pos[r * 4 + 0] = t.x;
-            pos[r * 4 + 1] = t.y;
-            pos[r * 4 + 2] = t.z;
-            pos[r * 4 + 3] = t.w;
-        }
-        return;
-    }
-    const int n[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int r0 = row[n[j] - 1], r1 = row[n[j]];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            int p = -1;
-            for (int l = r0; l < r1; l++)
-                if (col[l] == n[k]) { p = l; break; }
-            pos[j * 4 + k] = p;
-        }
-    }
+    if (e2e) return e2e[(int64_t)e * 16 + q];
+    for (int l = row[nj - 1]; l < row[nj]; l++)
+        if (col[l] == nk) return l;
+    return -1;
 }
 
+// One color: 16 lanes per element, lane q = 4j + k adds the element's (j,k) block into
+// its CSR entry. Elements of one color share no node, so no two lanes of a launch touch
+// the same entry; each entry receives its elements' blocks in color order, as in the
+// reference COLORING build. The 16 lanes of an element compute its gradients alike.
 template <int OP, int PREC>
 __global__ void __launch_bounds__(kBlock)
     k_assemble_color(const double *__restrict__ coord, const double *__restrict__ coef,
@@ -67,79 +51,51 @@ __global__ void __launch_bounds__(kBlock)
                      const int *__restrict__ col, const int4 *__restrict__ e2n,
                      const int *__restrict__ e2e, int first, int count)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    const int e = first + t;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)count * 16) return;
+    const int e = first + (int)(t >> 4), q = (int)(t & 15), j = q >> 2, k = q & 3;
     const int4 v = e2n[e];
     double c[4][3], g[4][3];
     mf::load_coords(coord, v, c);
     mf::elem_grad(c, g);
-    int pos[16];
-    elem_positions(row, col, e2e, e, v, pos);
+    const int n[4] = {v.x, v.y, v.z, v.w};
+    const int p = elem_position(row, col, e2e, e, n[j], n[k], q);
+    if (p < 0) return;
     if (OP == MF_OP_LAP) {
         const double kd = coef ? coef[e] : 1.0;
         if (PREC == MF_PREC_FP64) {
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int p = pos[j * 4 + k];
-                    if (p < 0) continue;
-                    double x = mf::lap_entry(g[j], g[k]);
-                    if (coef) x = kd * x;
-                    values[p] += x;
-                }
+            double x = mf::lap_entry(g[j], g[k]);
+            if (coef) x = kd * x;
+            values[p] += x;
         } else {
-            float gf[4][3];
+            float gj[3], gk[3];
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int d = 0; d < 3; d++) gf[i][d] = (float)g[i][d];
-            const float kf = (float)kd;
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int p = pos[j * 4 + k];
-                    if (p < 0) continue;
-                    const float x = kf * mf::lap_entry(gf[j], gf[k]);
-                    values[p] += (double)x;
-                }
+            for (int d = 0; d < 3; d++) {
+                gj[d] = (float)g[j][d];
+                gk[d] = (float)g[k][d];
+            }
+            const float x = (float)kd * mf::lap_entry(gj, gk);
+            values[p] += (double)x;
         }
     } else {
         const mf::ElaCoef kc = mf::ela_coef(coef, e);
+        double *out = values + (int64_t)p * 9;
         if (PREC == MF_PREC_FP64) {
+            double b[9];
+            mf::ela_block(g[j], g[k], kc.cd, kc.co, kc.cm, b);
 #pragma unroll
-            for (int j = 0; j < 4; j++)
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int p = pos[j * 4 + k];
-                    if (p < 0) continue;
-                    double b[9];
-                    mf::ela_block(g[j], g[k], kc.cd, kc.co, kc.cm, b);
-                    double *out = values + (int64_t)p * 9;
-#pragma unroll
-                    for (int q = 0; q < 9; q++) out[q] += b[q];
-                }
+            for (int r = 0; r < 9; r++) out[r] += b[r];
         } else {
-            float gf[4][3];
+            float gj[3], gk[3];
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int d = 0; d < 3; d++) {
+                gj[d] = (float)g[j][d];
+                gk[d] = (float)g[k][d];
+            }
+            float b[9];
+            mf::ela_block(gj, gk, (float)kc.cd, (float)kc.co, (float)kc.cm, b);
 #pragma unroll
-                for (int d = 0; d < 3; d++) gf[i][d] = (float)g[i][d];
-            const float cd = (float)kc.cd, co = (float)kc.co, cm = (float)kc.cm;
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int p = pos[j * 4 + k];
-                    if (p < 0) continue;
-                    float b[9];
-                    mf::ela_block(gf[j], gf[k], cd, co, cm, b);
-                    double *out = values + (int64_t)p * 9;
-#pragma unroll
-                    for (int q = 0; q < 9; q++) out[q] += (double)b[q];
-                }
+            for (int r = 0; r < 9; r++) out[r] += (double)b[r];
         }
     }
 }
@@ -239,7 +195,8 @@ int launch_colors(const double *coord, const double *coef, double *values, const
     for (int c = 0; c < nbColors; c++) {
         const int first = colorToElem[c], count = colorToElem[c + 1] - first;
         if (count <= 0) continue;
-        hipLaunchKernelGGL((k_assemble_color<OP, PREC>), dim3(grid_for(count)), dim3(kBlock),
+        hipLaunchKernelGGL((k_assemble_color<OP, PREC>), dim3(grid_for((int64_t)count * 16)),
+                           dim3(kBlock),
                            0, s, coord, coef, values, row, col,
                            reinterpret_cast<const int4 *>(e2n), e2e, first, count);
         MF_CHECK_LAUNCH();
