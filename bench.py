@@ -120,19 +120,24 @@ def main():
         log(f"plan: {plan.stats()} ({time.time() - t1:.1f}s)")
     stream = gpu.Stream()
 
+    out_bytes = 8 * mesh.nnz * (9 if op == minifem.OP_ELA else 1)
+
+    def probe_us(mode, steps):
+        """Mean time of the plan's output stream alone (mf_plan_write_probe)."""
+        for _ in range(args.warmup):
+            plan.write_probe(dm.values, mode, stream)
+        e0, e1 = gpu.Event(), gpu.Event()
+        e0.record(stream)
+        for _ in range(steps):
+            plan.write_probe(dm.values, mode, stream)
+        e1.record(stream)
+        return e0.elapsed_ms(e1) / steps * 1e3
+
     if args.write_probe:
-        out_bytes = 8 * mesh.nnz * (9 if op == minifem.OP_ELA else 1)
         res = {}
         for mode, name in ((0, "tile_rows"), (1, "front_to_back"), (2, "tile_rows_whole_lines"),
                            (3, "tile_rows_partial_lines")):
-            for _ in range(args.warmup):
-                plan.write_probe(dm.values, mode, stream)
-            e0, e1 = gpu.Event(), gpu.Event()
-            e0.record(stream)
-            for _ in range(args.steps):
-                plan.write_probe(dm.values, mode, stream)
-            e1.record(stream)
-            us = e0.elapsed_ms(e1) / args.steps * 1e3
+            us = probe_us(mode, args.steps)
             res[name] = {"us": round(us, 1)}
             if mode < 2:  # modes 2 and 3 write a part of the bytes each
                 res[name]["TB_per_s"] = round(out_bytes / us / 1e6, 3)
@@ -140,6 +145,13 @@ def main():
                           "workload": f"{args.op} {args.n}^3 node-shuffled, tiled plan "
                                       f"{plan.stats()['tiles']} tiles"}), flush=True)
         return
+
+    # the floor the values' own write order sets, measured before the timed steps (the
+    # probe overwrites the values; the warm-up steps assemble them again)
+    floor = None
+    if plan is not None:
+        floor = {"tile_rows_us": round(probe_us(0, 20), 1),
+                 "front_to_back_us": round(probe_us(1, 20), 1), "bytes": out_bytes}
 
     def step():
         if plan is not None:
@@ -181,6 +193,9 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": kname, "algorithmic_bytes_per_launch": B,
                 "mean_launch_us": round(kern_s * 1e6, 2)}
+    if floor is not None:
+        floor["kernel_over_tile_rows"] = round(kern_s * 1e6 / floor["tile_rows_us"], 3)
+        roofline["output_write_floor"] = floor
     log(f"GPU: {ms_step * 1e3:.1f} us/step, min {per.min() * 1e3:.1f} us, "
         f"{achieved:.0f} GB/s algorithmic ({100 * achieved / HBM_PEAK_GBS:.1f}% of peak)")
 

@@ -13,7 +13,7 @@
 //      - a row's incident elements are kept in LDS as an ascending "incidence list";
 //      - an off-diagonal block (row a, column c) is one lane, which reads the elements
 //        holding both a and c, with a's and c's vertex slots in them, from its task
-//        record (or, past 8 elements, walks a mask over row a's incidence list);
+//        record (or, past 6 elements, walks a mask over row a's incidence list);
 //        block (c,a) is its exact transpose (same elements in the same order, and
 //        IEEE products commute), so a pair is computed once and written twice when
 //        c is a row of the same tile (a column of another tile: that tile computes
@@ -133,7 +133,7 @@ struct TileHdr {
     int32_t n_rows;
     int32_t rinfo_start;         // into tile_rowinfo: n_rows entries
     int32_t inc_start, n_inc;    // into inc (multiples of 8)
-    int32_t task_start, n_task;  // into task_info / task_xpos / task_rec
+    int32_t task_start, n_task;  // into task_rec (4 words per task)
     int32_t n_ent;               // CSR entries of the tile's rows
     int32_t pad;
 };
