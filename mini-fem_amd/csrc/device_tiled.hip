@@ -129,7 +129,8 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
     constexpr int NPT = (mf::kMaxTileNodes + LT - 1) / LT;               // loader: nodes, rows
     constexpr int EPT = (MAXE + LT - 1) / LT;                            // loader: elements
     constexpr int IPT = (mf::kMaxTileInc / 8 + LT - 1) / LT;             // loader: 16-byte incidences
-    constexpr int TPT = (mf::max_tile_blocks(BT) * 64 + LT - 1) / LT;    // loader: tasks
+    constexpr int TPT = (mf::max_tile_tasks(ELA, BT) + LT - 1) / LT;     // loader: tasks
+    constexpr int TB = mf::task_blocks_per_wave(ELA);                   // task blocks per wave
     static_assert(mf::kMaxTileRows <= mf::kMaxTileNodes, "rows are the first tile nodes");
     extern __shared__ __align__(16) unsigned char lds_raw[];
 
@@ -393,11 +394,19 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
         double blk[NA];
 #pragma unroll
         for (int q = 0; q < NA; q++) blk[q] = 0.0;
-        const int m = wave - nDB;                // task block
-        const int ti = m * 64 + lane;
-        const bool tOn = !isDiag && m < nEB && ti < h.n_task;
-        int tEnt = 0;                            // tile entry of the task's block
-        int xEnt = -1;                           // tile entry of its transpose, or -1
+        // task blocks of this wave: (wave - nDB) + k * (NC - nDB), k < TB
+        bool tOn[TB];
+        int tEnt[TB], xEnt[TB];  // tile entries of the task's block and of its transpose (-1: none)
+        double tres[TB][NA];
+#pragma unroll
+        for (int k = 0; k < TB; k++) {
+            const int m = (wave - nDB) + k * (NC - nDB);
+            tOn[k] = !isDiag && m < nEB && m * 64 + lane < h.n_task;
+            tEnt[k] = 0;
+            xEnt[k] = -1;
+#pragma unroll
+            for (int q = 0; q < NA; q++) tres[k][q] = 0.0;
+        }
         if (isDiag) {
             // the diagonal chains are the tile's critical path: this wave's instructions
             // go first on its SIMD while it sums them
@@ -477,15 +486,20 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                 blk[0] = s0;
             }
             __builtin_amdgcn_s_setprio(0);
-        } else if (tOn) {
+        } else {
+#pragma unroll
+        for (int kb = 0; kb < TB; kb++) {
+            if (!tOn[kb]) continue;
             // off-diagonal task: the elements holding both a and c, two per iteration
             // (independent LDS chains, summed in order); the block goes to row a, its
             // transpose to row c
+            const int ti = ((wave - nDB) + kb * (NC - nDB)) * 64 + lane;
+            double (&acc)[NA] = tres[kb];
             const uint4 rc = ltask[ti];
             const int ra = rc.w & 0xFF, oa = (rc.w >> 8) & 0xFF, cl = (rc.w >> 16) & 0xFF;
             const int4 ri = lri[ra];
-            tEnt = ri.w + oa;
-            if (cl < R) xEnt = lri[cl].w + (int)(rc.w >> 24);  // c is a row of this tile
+            tEnt[kb] = ri.w + oa;
+            if (cl < R) xEnt[kb] = lri[cl].w + (int)(rc.w >> 24);  // c is a row of this tile
             // one contribution pair (element slot, vertex slots of a and c); a padding
             // entry reads the all-zero slot TE, whose products are +0.0 and leave the
             // sums unchanged
@@ -513,10 +527,10 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                         T bl[9];
                         mf::ela_block(ga[u], gk[u], c0[u], c1[u], c2[u], bl);
 #pragma unroll
-                        for (int q = 0; q < 9; q++) blk[q] += (double)bl[q];
+                        for (int q = 0; q < 9; q++) acc[q] += (double)bl[q];
                     } else {
                         const T x = c0[u] * mf::lap_entry(ga[u], gk[u]);
-                        blk[0] += (double)x;
+                        acc[0] += (double)x;
                     }
                 }
             };
@@ -555,6 +569,7 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                 }
             }
         }
+        }
         __syncthreads();  // 2: every stage read is done
 
         // ---- O: the blocks (and in-tile transposes) into the output image
@@ -562,19 +577,23 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
             double *o = outv + (int64_t)(dri.w + ((dri.y >> 16) & 0xFF)) * NA;
 #pragma unroll
             for (int q = 0; q < NA; q++) o[q] = blk[q];
-        } else if (tOn) {
-            double *o = outv + (int64_t)tEnt * NA;
+        }
 #pragma unroll
-            for (int q = 0; q < NA; q++) o[q] = blk[q];
-            if (xEnt >= 0) {
+        for (int kb = 0; kb < TB; kb++) {
+            if (!tOn[kb]) continue;
+            const double (&r)[NA] = tres[kb];
+            double *o = outv + (int64_t)tEnt[kb] * NA;
+#pragma unroll
+            for (int q = 0; q < NA; q++) o[q] = r[q];
+            if (xEnt[kb] >= 0) {
                 if constexpr (ELA) {
-                    const double tr[9] = {blk[0], blk[3], blk[6], blk[1], blk[4],
-                                          blk[7], blk[2], blk[5], blk[8]};  // transpose
-                    double *ot = outv + (int64_t)xEnt * 9;
+                    const double tr[9] = {r[0], r[3], r[6], r[1], r[4],
+                                          r[7], r[2], r[5], r[8]};  // transpose
+                    double *ot = outv + (int64_t)xEnt[kb] * 9;
 #pragma unroll
                     for (int q = 0; q < 9; q++) ot[q] = tr[q];
                 } else {
-                    outv[xEnt] = blk[0];
+                    outv[xEnt[kb]] = r[0];
                 }
             }
         }

@@ -68,11 +68,15 @@ struct Partitioner {
         if (NN > kMaxTileNodes || TE > max_tile_elems(block)) return false;
         const int64_t inc8 = (inc + 7) / 8 * 8;
         if (inc8 > kMaxTileInc || ent > kMaxLdsBytes) return false;
-        if (diag_blocks(ela, R) + (tasks + kLanes - 1) / kLanes > max_tile_blocks(block)) return false;
+        if (!tile_blocks_fit(ela, block, diag_blocks(ela, R), (int)((tasks + kLanes - 1) / kLanes)))
+            return false;
         return tile_lds_layout(ela, (int)NN, (int)TE, (int)TE, R, (int)inc8, (int)tasks, (int)ent,
                                slot).total <= budget;
     }
 
+    // Recursive coordinate bisection along the longest axis. A region whose halves do
+    // not fit but whose thirds do is cut in three: tiles of a third rather than a quarter
+    // of it, i.e. fewer and larger tiles.
     void rcb(int lo, int hi)
     {
         if (hi - lo <= 1 || fits(lo, hi)) {
@@ -89,13 +93,26 @@ struct Partitioner {
         int ax = 0;
         for (int d = 1; d < 3; d++)
             if (mx[d] - mn[d] > mx[ax] - mn[ax]) ax = d;
-        const int mid = lo + (hi - lo) / 2;
         const double *c = coord;
-        std::nth_element(order.begin() + lo, order.begin() + mid, order.begin() + hi,
-                         [c, ax](int a, int b) {
-                             const double xa = c[(int64_t)a * 3 + ax], xb = c[(int64_t)b * 3 + ax];
-                             return xa < xb || (xa == xb && a < b);
-                         });
+        auto less = [c, ax](int a, int b) {
+            const double xa = c[(int64_t)a * 3 + ax], xb = c[(int64_t)b * 3 + ax];
+            return xa < xb || (xa == xb && a < b);
+        };
+        const int mid = lo + (hi - lo) / 2;
+        std::nth_element(order.begin() + lo, order.begin() + mid, order.begin() + hi, less);
+        if (hi - lo >= 6 && !(fits(lo, mid) && fits(mid, hi))) {
+            const int t1 = lo + (hi - lo) / 3, t2 = lo + 2 * (hi - lo) / 3;
+            std::vector<int> saved(order.begin() + lo, order.begin() + hi);
+            std::nth_element(order.begin() + lo, order.begin() + t1, order.begin() + hi, less);
+            std::nth_element(order.begin() + t1, order.begin() + t2, order.begin() + hi, less);
+            if (fits(lo, t1) && fits(t1, t2) && fits(t2, hi)) {
+                leaves.emplace_back(lo, t1);
+                leaves.emplace_back(t1, t2);
+                leaves.emplace_back(t2, hi);
+                return;
+            }
+            std::copy(saved.begin(), saved.end(), order.begin() + lo);  // back to the halves
+        }
         rcb(lo, mid);
         rcb(mid, hi);
     }
@@ -359,7 +376,7 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
             return MF_ERR_STATE;
         // one work block per compute wave, and a compute wave without a diagonal block
         const int nDB = diag_blocks(ela, R), nEB = (h.n_task + kLanes - 1) / kLanes;
-        if (nDB + nEB > max_tile_blocks(p.block) || nDB >= compute_waves(p.block)) return MF_ERR_STATE;
+        if (!tile_blocks_fit(ela, p.block, nDB, nEB)) return MF_ERR_STATE;
         if (h.n_ent < R || h.n_node > p.max_node || h.n_elem > p.max_elem ||
             tile_lds_layout(ela, p.max_node, p.max_elem, h.n_elem, R, h.n_inc, h.n_task, h.n_ent,
                             slot_bytes).total + 16 > kMaxLdsBytes)

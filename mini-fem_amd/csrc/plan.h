@@ -64,9 +64,21 @@ MF_HOST_DEVICE constexpr int max_tile_elems(int block)
     return kMaxTileElems < 2 * kLanes * compute_waves(block) ? kMaxTileElems
                                                                : 2 * kLanes * compute_waves(block);
 }
-// Work blocks of 64 lanes (diagonal + task) per tile: one per compute wave, and at least
-// one compute wave left for the copy-out while the diagonal waves invert.
-MF_HOST_DEVICE constexpr int max_tile_blocks(int block) { return compute_waves(block); }
+// Work blocks of 64 lanes per tile. A diagonal block takes a compute wave; the other
+// compute waves (at least one, which also copies the output out while the diagonal waves
+// invert) take up to task_blocks_per_wave task blocks each: one for ela (a 3x3 block
+// per lane held in registers), two for lap (one double per lane).
+MF_HOST_DEVICE constexpr int task_blocks_per_wave(bool ela) { return ela ? 1 : 2; }
+MF_HOST_DEVICE constexpr bool tile_blocks_fit(bool ela, int block, int n_diag, int n_task_blocks)
+{
+    return n_diag < compute_waves(block) &&
+           n_task_blocks <= task_blocks_per_wave(ela) * (compute_waves(block) - n_diag);
+}
+// Task records one tile holds at most.
+MF_HOST_DEVICE constexpr int max_tile_tasks(bool ela, int block)
+{
+    return task_blocks_per_wave(ela) * compute_waves(block) * kLanes;
+}
 
 // Element-stage slot of one tile element, in units of the stage type (double or float).
 //   ela: vertex v's {gx, gy, gz, lambda+2mu} at 4v, then {lambda+mu, mu} at 16: a vertex's
@@ -125,7 +137,8 @@ MF_HOST_DEVICE inline TileLds tile_lds_layout(bool ela, int f_node, int f_elem, 
 MF_HOST_DEVICE inline int diag_blocks(bool, int n_rows) { return (n_rows + kLanes - 1) / kLanes; }
 
 // Work blocks of a tile on its compute waves: diagonal block i (rows 64i..) on compute
-// wave i, task block m (tasks 64m..) on compute wave diag_blocks + m.
+// wave i; task block m (tasks 64m..) on compute wave diag_blocks + m % w, as its
+// (m / w)-th task block, w = compute waves - diag_blocks.
 
 struct TileHdr {
     int32_t elem_start, n_elem;  // into tile_elems / tile_verts (ascending element ids)
