@@ -3,10 +3,10 @@ CPU oracle, through the C ABI.
 
 Bar: values and prec bit-exact against the oracle's COLORING-order restatement for
 FP64; mixed precision bit-exact against the oracle's mixed restatement and within
-1e-6 relative Frobenius error of FP64. At the 100^3 size the oracle is not run;
-size-independent properties are checked instead (exact block symmetry, vanishing row
-sums, prec times the masked diagonal block = I, agreement with the drop-in colored
-path bit for bit).
+1e-6 relative Frobenius error of FP64. The benchmark workload itself (ela 100^3) is
+checked bit for bit against the oracle; at 100^3 size-independent properties are checked
+as well (exact block symmetry, vanishing row sums, prec times the masked diagonal block
+= I, agreement with the drop-in colored path bit for bit).
 """
 import numpy as np
 import pytest
@@ -238,3 +238,24 @@ def test_write_probe_covers_every_value(mf, orc, gpu, op):
         plan.write_probe(buf, 2)
         part = buf.numpy()[:n]
         assert np.any(np.isnan(part)) and np.any(part == 0.0)
+
+
+def test_tiled_bit_exact_bench_workload(mf, orc, gpu):
+    """The benchmark workload itself (ela 100^3, jitter 0.25h, seeds 3/103/203, node ids
+    shuffled) under the benchmark's schedule (155000 B tiles, 1024 threads, 254 rows):
+    FP64 values and prec bit-exact against the oracle, and the mixed form bit-exact against
+    the oracle's mixed restatement."""
+    print("[tiled bench workload] ela 100^3", flush=True)
+    coord, e2n, code, nnz = mf.generate_cube(100, 0.25, 3, 203)
+    coef = mf.generate_coef(1, e2n.size // 4, 103)
+    mesh = mf.Mesh(coord, e2n, code, nnz, 1, coef=coef)
+    plan, dm = run_plan(gpu, mesh, tile_lds_bytes=155000, block_threads=1024, max_tile_rows=254)
+    vals, prec = dm.values.numpy(), dm.prec.numpy()
+    om = orc.OracleMesh(coord, e2n, code, nnz, 1, coef=coef)
+    ov = om.assemble()
+    assert np.array_equal(vals, ov)
+    assert np.array_equal(prec, om.prec(ov)[0])
+    del ov, prec
+    dm.values.fill(np.nan)
+    plan.assemble(dm.coord, dm.coef, dm.values, dm.prec, precision=mf.PREC_MIXED)
+    assert np.array_equal(dm.values.numpy(), om.assemble(mixed=True))
