@@ -248,7 +248,9 @@ def main():
 
 
 def bench_batched(args):
-    """64 independent meshes assembled by one replay of the captured batch graph."""
+    """64 independent meshes assembled by one replay of the captured batch graph, in the
+    per-mesh form (one launch per mesh on 8 side streams) and the union form (one
+    persistent launch per operator over the disjoint union of its meshes)."""
     import minifem
     import minifem.gpu as gpu
     from minifem.batched import Batch, batch_meshes, tune_batch
@@ -257,21 +259,28 @@ def bench_batched(args):
     meshes = batch_meshes()
     log(f"batch: {len(meshes)} meshes, {sum(m.nbNodes for m in meshes)} nodes, "
         f"{sum(m.nnz for m in meshes)} blocks ({time.time() - t0:.1f}s)")
-    best, res = tune_batch(meshes, log=log)
     out = {"metric": "batched CSR blocks assembled/s (64 meshes, one graph launch)",
-           "unit": "CSR blocks/s", "n_gpus": 1, "data": "synthetic", "schedule": best}
-    for name, prec in (("fp64", minifem.PREC_FP64), ("mixed", minifem.PREC_MIXED)):
-        b = Batch(meshes, precision=prec, **best).capture()
-        ms_graph = b.time_ms(iters=args.steps, warmup=args.warmup)
-        ms_eager = b.time_ms(iters=args.steps, warmup=args.warmup, graph=False)
-        out[name] = {"ms_per_batch": round(ms_graph, 4), "ms_per_batch_eager": round(ms_eager, 4),
-                     "value": round(b.blocks() / (ms_graph * 1e-3), 1)}
-        del b
+           "unit": "CSR blocks/s", "n_gpus": 1, "data": "synthetic"}
     B = sum(algorithmic_bytes(m.nbElem, m.nbNodes, m.nnz, m.operatorID) for m in meshes)
-    out["roofline_fp64"] = {"achieved": round(B / (out["fp64"]["ms_per_batch"] * 1e-3) / 1e9, 1),
-                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(B / (out["fp64"]["ms_per_batch"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    out["value"] = out["fp64"]["value"]
+    for form, union in (("per_mesh", False), ("union", True)):
+        best, res = tune_batch(meshes, log=log, union=union)
+        out[form] = {"schedule": best}
+        for name, prec in (("fp64", minifem.PREC_FP64), ("mixed", minifem.PREC_MIXED)):
+            b = Batch(meshes, precision=prec, union=union, **best).capture()
+            ms_graph = b.time_ms(iters=args.steps, warmup=args.warmup)
+            ms_eager = b.time_ms(iters=args.steps, warmup=args.warmup, graph=False)
+            out[form][name] = {"ms_per_batch": round(ms_graph, 4),
+                               "ms_per_batch_eager": round(ms_eager, 4),
+                               "launches": b.launches(),
+                               "value": round(b.blocks() / (ms_graph * 1e-3), 1)}
+            log(f"{form} {name}: {out[form][name]}")
+            del b
+        gbs = B / (out[form]["fp64"]["ms_per_batch"] * 1e-3) / 1e9
+        out[form]["roofline_fp64"] = {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    out["union_vs_per_mesh_fp64"] = round(out["union"]["fp64"]["value"] /
+                                          out["per_mesh"]["fp64"]["value"], 3)
+    out["value"] = max(out["union"]["fp64"]["value"], out["per_mesh"]["fp64"]["value"])
     print(json.dumps(out), flush=True)
 
 

@@ -2,10 +2,19 @@
 mesh's fused iteration (mf_plan_assemble: assembly + block-Jacobi prec,
 src/FEM.cc:183-232) captured into ONE HIP graph and replayed as one submission.
 
-Inside the graph the meshes are dealt round-robin to `branches` parallel streams
-(forked from and joined back to the capturing stream through events), so small
-meshes that cannot fill the GPU on their own run side by side. The schedule parameters
-of the tiled plans are picked for the whole batch by `tune_batch`.
+Two forms:
+  - per mesh (union=False): one plan and one launch per mesh, the meshes dealt
+    round-robin to `branches` parallel streams inside the graph (forked from and joined
+    back to the capturing stream through events);
+  - union (union=True): the meshes of one operator become ONE disjoint-union mesh with
+    one plan, so the whole batch is one persistent launch per operator (two side streams
+    when both operators are present). The workgroups of that launch take tiles of every
+    mesh, so small meshes never leave the GPU idle and no launch waits for another.
+    Results are bit-identical to the per-mesh form: the greedy coloring of a disjoint
+    union colors every element as its own mesh does, and its color permutation keeps
+    each mesh's elements in their own relative order, so every CSR block sums the same
+    products in the same order.
+The schedule parameters of the tiled plans are picked for the whole batch by `tune_batch`.
 """
 import numpy as np
 
@@ -30,16 +39,87 @@ def batch_meshes(count=64, n_lo=5000, n_hi=200000, seed=1000, jitter=0.2):
     return meshes
 
 
+class UnionMesh:
+    """The disjoint union of meshes of one operator as one Mesh: mesh k's nodes are
+    node_off[k] .. node_off[k+1]-1 of the union, its elements keep their input order
+    after those of meshes 0..k-1, its coefficients and boundary codes come along.
+
+    partition_coord: the union's coordinates with the meshes set side by side along x
+    (a gap of `gap` times the widest mesh between neighbours), used only to group nodes
+    into tiles, so a tile holds nodes of one mesh except where a cut falls between two.
+    """
+
+    def __init__(self, meshes, gap=0.5):
+        self.parts = list(meshes)
+        if not self.parts:
+            raise ValueError("UnionMesh needs at least one mesh")
+        op = self.parts[0].operatorID
+        if any(m.operatorID != op for m in self.parts):
+            raise ValueError("UnionMesh: all meshes must have the same operator")
+        have_coef = [m.coef_original is not None for m in self.parts]
+        if any(have_coef) and not all(have_coef):
+            raise ValueError("UnionMesh: either every mesh or none has coefficients")
+        n = np.array([m.nbNodes for m in self.parts], np.int64)
+        self.node_off = np.concatenate([[0], np.cumsum(n)])
+        if self.node_off[-1] >= 2**31 - 1:
+            raise ValueError("UnionMesh: too many nodes for 32-bit node ids")
+        e2n = np.concatenate([m.elemToNode_original.astype(np.int64) + self.node_off[k]
+                              for k, m in enumerate(self.parts)]).astype(np.int32)
+        coord = np.concatenate([m.coord for m in self.parts])
+        code = np.concatenate([m.boundNodesCode for m in self.parts])
+        coef = (np.concatenate([m.coef_original for m in self.parts])
+                if all(have_coef) else None)
+        cap = sum(m.nnz for m in self.parts)
+        self.mesh = Mesh(coord, e2n, code, cap, op, coef=coef)
+        if self.mesh.nnz != cap:
+            raise AssertionError("UnionMesh: CSR size differs from the sum of the parts")
+        pc = coord.reshape(-1, 3).copy()
+        lo = np.array([m.coord.reshape(-1, 3)[:, 0].min() for m in self.parts])
+        hi = np.array([m.coord.reshape(-1, 3)[:, 0].max() for m in self.parts])
+        step = (hi - lo).max() * (1.0 + gap) + 1e-12
+        for k in range(len(self.parts)):
+            a, b = self.node_off[k], self.node_off[k + 1]
+            pc[a:b, 0] += k * step - lo[k]
+        self.partition_coord = pc.ravel()
+
+    def value_range(self, k):
+        """Mesh k's CSR values in the union's value array (doubles)."""
+        r, d = self.mesh.row, self.mesh.dim
+        return int(r[self.node_off[k]]) * d, int(r[self.node_off[k + 1]]) * d
+
+    def prec_range(self, k):
+        """Mesh k's prec blocks in the union's prec array (doubles)."""
+        d = self.mesh.dim
+        return int(self.node_off[k]) * d, int(self.node_off[k + 1]) * d
+
+
 class Batch:
     """Device-resident meshes with their plans, and the captured batch graph."""
 
-    def __init__(self, meshes, precision=PREC_FP64, branches=8, **plan_params):
+    def __init__(self, meshes, precision=PREC_FP64, branches=8, union=False, **plan_params):
         self.meshes = list(meshes)
         self.precision = precision
-        self.dms = [gpu.DeviceMesh(m) for m in self.meshes]
-        self.plans = [gpu.Plan(m, **plan_params) for m in self.meshes]
+        self.union = bool(union)
+        self._where = None
+        if self.union:
+            # One union per operator, in order of first appearance; _where[i] = (union,
+            # part) of input mesh i.
+            groups = {}
+            for i, m in enumerate(self.meshes):
+                groups.setdefault(m.operatorID, []).append(i)
+            self.unions = [UnionMesh([self.meshes[i] for i in idx]) for idx in groups.values()]
+            self._where = [None] * len(self.meshes)
+            for u, idx in enumerate(groups.values()):
+                for k, i in enumerate(idx):
+                    self._where[i] = (u, k)
+            self.dms = [gpu.DeviceMesh(u.mesh) for u in self.unions]
+            self.plans = [gpu.Plan(u.mesh, partition_coord=u.partition_coord, **plan_params)
+                          for u in self.unions]
+        else:
+            self.dms = [gpu.DeviceMesh(m) for m in self.meshes]
+            self.plans = [gpu.Plan(m, **plan_params) for m in self.meshes]
         self.stream = gpu.Stream()
-        self.branches = max(1, min(int(branches), len(self.meshes)))
+        self.branches = max(1, min(int(branches), len(self.plans)))
         self._side = [gpu.Stream() for _ in range(self.branches)]
         self._fork = gpu.Event()
         self._join = [gpu.Event() for _ in range(self.branches)]
@@ -90,8 +170,20 @@ class Batch:
         """CSR blocks assembled per batch iteration."""
         return sum(m.nnz for m in self.meshes)
 
+    def launches(self):
+        """Assembly launches per batch iteration."""
+        return len(self.plans)
+
     def results(self):
-        return [(dm.values.numpy(), dm.prec.numpy()) for dm in self.dms]
+        """(values, prec) of every input mesh, in input order."""
+        if not self.union:
+            return [(dm.values.numpy(), dm.prec.numpy()) for dm in self.dms]
+        full = [(dm.values.numpy(), dm.prec.numpy()) for dm in self.dms]
+        out = []
+        for u, k in self._where:
+            (va, vb), (pa, pb) = self.unions[u].value_range(k), self.unions[u].prec_range(k)
+            out.append((full[u][0][va:vb].copy(), full[u][1][pa:pb].copy()))
+        return out
 
 
 BATCH_CANDIDATES = [
@@ -100,12 +192,13 @@ BATCH_CANDIDATES = [
 ]
 
 
-def tune_batch(meshes, precision=PREC_FP64, candidates=None, branches=8, iters=5, log=None):
+def tune_batch(meshes, precision=PREC_FP64, candidates=None, branches=8, iters=5, log=None,
+               union=False):
     """Times the captured batch under each candidate schedule (one parameter set for all
     meshes) and returns (best_params, [(params, ms), ...])."""
     results = []
     for c in candidates or BATCH_CANDIDATES:
-        b = Batch(meshes, precision=precision, branches=branches, **c).capture()
+        b = Batch(meshes, precision=precision, branches=branches, union=union, **c).capture()
         ms = b.time_ms(iters=iters)
         results.append((c, ms))
         if log:

@@ -327,17 +327,23 @@ class Plan:
     (mf_plan_create). assemble() runs one fused iteration: values + inverted prec."""
 
     def __init__(self, mesh, tile_lds_bytes=0, block_threads=0, max_tile_rows=0,
-                 use_coord=True):
+                 use_coord=True, partition_coord=None):
+        """partition_coord: coordinates used only to group nodes into tiles (default: the
+        mesh's own)."""
         from ._lib import PlanParams
         _ensure()
         self._lib = lib()
         self.mesh = mesh
         params = PlanParams(int(tile_lds_bytes), int(block_threads), int(max_tile_rows))
+        pc = None
+        if use_coord:
+            pc = mesh.coord if partition_coord is None else np.ascontiguousarray(
+                partition_coord, np.float64)
         p = ctypes.c_void_p()
         check(self._lib.mf_plan_create(
             ctypes.byref(p), mesh.operatorID, mesh.nbElem, mesh.nbNodes, mesh.nnz,
             mesh.elemToNode.ctypes.data, mesh.row.ctypes.data, mesh.col.ctypes.data,
-            mesh.checkBounds.ctypes.data, mesh.coord.ctypes.data if use_coord else None,
+            mesh.checkBounds.ctypes.data, None if pc is None else pc.ctypes.data,
             ctypes.byref(params)), "mf_plan_create")
         self.ptr = p.value
 

@@ -218,15 +218,18 @@ class Mesh:
         return cls(coord, e2n, code, nnz, operator_id, coef=coef, colored=colored)
 
 
-def plan_host_stats(mesh, tile_lds_bytes=0, block_threads=0, max_tile_rows=0, use_coord=True):
-    """Builds and validates the tiled gather schedule on the host (mf_plan_host_stats)."""
+def plan_host_stats(mesh, tile_lds_bytes=0, block_threads=0, max_tile_rows=0, use_coord=True,
+                    partition_coord=None):
+    """Builds and validates the tiled gather schedule on the host (mf_plan_host_stats).
+    partition_coord: coordinates used only to group nodes into tiles (default: the mesh's)."""
     import ctypes
     from ._lib import PlanParams
     params = PlanParams(int(tile_lds_bytes), int(block_threads), int(max_tile_rows))
     out = np.zeros(8, np.int64)
+    pc = mesh.coord if partition_coord is None else _c(partition_coord, np.float64)
     check(lib().mf_plan_host_stats(mesh.operatorID, mesh.nbElem, mesh.nbNodes, mesh.nnz,
                                    _ptr(mesh.elemToNode), _ptr(mesh.row), _ptr(mesh.col),
-                                   _ptr(mesh.coord) if use_coord else None,
+                                   _ptr(pc) if use_coord else None,
                                    ctypes.byref(params), out.ctypes.data), "mf_plan_host_stats")
     keys = ["tiles", "tile_elems", "tasks", "incidences", "plan_bytes", "lds_bytes",
             "block_threads", "max_rows"]

@@ -2,7 +2,8 @@
 
 Batched: the captured batch graph, replayed several times with the program's own eager
 work and synchronisations in between, reproduces the eager per-mesh results bit for bit
-on every replay; one member against the oracle; mixed precision within 1e-6 of FP64.
+on every replay; the union form (one launch per operator) equals the per-mesh form bit
+for bit; one member against the oracle; mixed precision within 1e-6 of FP64.
 
 Serving: every request's (values, prec) served in the mix -- overlapping requests on
 several streams under an HBM budget that forces admission to wait -- equals the same
@@ -46,6 +47,34 @@ def test_batched_graph_replays_equal_eager(mf, orc, gpu):
     ov = om.assemble()
     assert np.array_equal(eager[1][0], ov)
     assert np.array_equal(eager[1][1], om.prec(ov)[0])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batched_union_equals_per_mesh(mf, gpu, precision):
+    """The union form (one persistent launch per operator) against the per-mesh form:
+    every mesh's values and prec bit for bit, on every graph replay."""
+    from minifem.batched import Batch
+    print(f"[batched] union vs per mesh, precision {precision}", flush=True)
+    meshes = small_batch(mf)
+    ref = Batch(meshes, precision=precision, branches=4)
+    ref.run_eager()
+    want = ref.results()
+    del ref
+    for params in (dict(), dict(tile_lds_bytes=150000, block_threads=1024, max_tile_rows=254)):
+        u = Batch(meshes, precision=precision, union=True, **params)
+        assert u.launches() == 2
+        u.capture()
+        for rep in range(2):
+            for dm in u.dms:
+                dm.values.fill(np.nan)
+                dm.prec.fill(np.nan)
+            u.launch()
+            u.stream.sync()
+            for i, ((v0, p0), (v1, p1)) in enumerate(zip(want, u.results())):
+                assert np.array_equal(v0, v1) and np.array_equal(p0, p1), (params, rep, i)
+            u.run_eager()
+            gpu.synchronize()
+        del u
 
 
 def test_batched_mixed_precision_error(mf, gpu):
