@@ -615,19 +615,34 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
         } else {
             const int nwr = NC - nDB, wr = wave - nDB;
             if constexpr (ELA) {
-                // one row per wave: 9 len doubles (~135) as 16-byte pieces, one 8-byte
-                // piece at the end of an odd-length row (8-byte aligned 16-byte stores
-                // are valid on gfx950; the LDS side reads two 8-byte words)
-                for (int rr = wr; rr < R; rr += nwr) {
+                // four rows per wave at a time, 16 lanes each. A row is 9 len doubles
+                // (~135) moved as 16-byte pieces, one 8-byte piece at the end of an
+                // odd-length row (8-byte aligned 16-byte stores are valid on gfx950; the
+                // LDS side reads two 8-byte words). A lane reads all its pieces of a pass
+                // (CP of them, 32 CP doubles per row and pass) before it stores any, so
+                // a row costs one LDS round trip, not one per piece.
+                constexpr int CP = 5;
+                const int g = lane >> 4, l16 = lane & 15;
+                for (int rr = wr * 4 + g; rr < R; rr += nwr * 4) {
                     const int4 ri = lri[rr];
                     const int n = (ri.y & 0xFFFF) * 9;
                     const double *src = outv + (int64_t)ri.w * 9;
                     double *dst = values + (int64_t)ri.x * 9;
-                    for (int j = 2 * lane; j < n; j += 128) {
-                        if (j + 1 < n)
-                            *reinterpret_cast<double2 *>(dst + j) = make_double2(src[j], src[j + 1]);
-                        else
-                            dst[j] = src[j];
+                    for (int j0 = 2 * l16; j0 < n; j0 += 32 * CP) {
+                        double2 v[CP];
+#pragma unroll
+                        for (int k = 0; k < CP; k++) {
+                            const int j = j0 + 32 * k;
+                            v[k] = make_double2(j < n ? src[j] : 0.0, j + 1 < n ? src[j + 1] : 0.0);
+                        }
+#pragma unroll
+                        for (int k = 0; k < CP; k++) {
+                            const int j = j0 + 32 * k;
+                            if (j + 1 < n)
+                                *reinterpret_cast<double2 *>(dst + j) = v[k];
+                            else if (j < n)
+                                dst[j] = v[k].x;
+                        }
                     }
                 }
             } else {
