@@ -11,8 +11,10 @@ inverted block-Jacobi prec, with all inputs resident in HBM.
 
 Prints one JSON line: metric/value (CSR blocks assembled per second), the roofline of
 the fused kernel (algorithmic HBM bytes per launch / its mean launch time, against the
-8 TB/s HBM3E peak), and the host-core baseline (the CPU oracle's REF-order iteration on
-the same mesh, timed here).
+8 TB/s HBM3E peak), the host-core baselines (the CPU oracle's REF-order iteration and
+its COLORING-order OpenMP iteration on the same mesh, timed here), and the accuracy of
+the GPU results against that oracle iteration (or, with --no-cpu in mixed precision,
+against an FP64 step).
 """
 import argparse
 import json
@@ -153,12 +155,12 @@ def main():
         floor = {"tile_rows_us": round(probe_us(0, 20), 1),
                  "front_to_back_us": round(probe_us(1, 20), 1), "bytes": out_bytes}
 
-    def step():
+    def step(precision=prec_mode):
         if plan is not None:
-            plan.assemble(dm.coord, dm.coef, dm.values, dm.prec, precision=prec_mode,
+            plan.assemble(dm.coord, dm.coef, dm.values, dm.prec, precision=precision,
                           stream=stream)
         else:
-            dm.iterate_colored(precision=prec_mode, stream=stream)
+            dm.iterate_colored(precision=precision, stream=stream)
 
     for _ in range(args.warmup):
         step()
@@ -199,7 +201,9 @@ def main():
     log(f"GPU: {ms_step * 1e3:.1f} us/step, min {per.min() * 1e3:.1f} us, "
         f"{achieved:.0f} GB/s algorithmic ({100 * achieved / HBM_PEAK_GBS:.1f}% of peak)")
 
-    cpu = cpu_omp = None
+    # the timed precision's results (the last step's output), checked below
+    res_v, res_p = dm.values.numpy(), dm.prec.numpy()
+    cpu = cpu_omp = accuracy = None
     if not args.no_cpu:
         import pyoracle
         t2 = time.perf_counter()
@@ -217,9 +221,9 @@ def main():
         # the reference's OpenMP COLORING build, restated: each color's elements in parallel
         threads = pyoracle.num_threads()
         t3 = time.perf_counter()
-        _, _, tus2 = pyoracle.fem_loop(mesh.coord, mesh.row, mesh.col, mesh.elemToNode,
-                                       mesh.checkBounds, mesh.colorToElem, op, mesh.nbNodes, 1, 2,
-                                       coef=mesh.coef)
+        ov, opv, tus2 = pyoracle.fem_loop(mesh.coord, mesh.row, mesh.col, mesh.elemToNode,
+                                          mesh.checkBounds, mesh.colorToElem, op, mesh.nbNodes,
+                                          1, 2, coef=mesh.coef)
         omp_s = float(np.sum(tus2)) * 1e-6
         log(f"CPU oracle COLORING/OpenMP ({threads} threads): {omp_s:.2f} s/iteration "
             f"({time.perf_counter() - t3:.1f}s)")
@@ -228,6 +232,23 @@ def main():
                    "sample": f"1 full iteration, COLORING order, elements of a color in parallel "
                              f"(OpenMP, {threads} threads, -O2) on the same mesh: {omp_s:.3f} s",
                    "gpu_speedup": round(value / (mesh.nnz / omp_s), 1)}
+        # the GPU results against the oracle's FP64 iteration just timed
+        accuracy = {"vs": "oracle FP64 iteration, COLORING order, same mesh (the CPU leg)",
+                    "values_bit_exact": bool(np.array_equal(res_v, ov)),
+                    "prec_bit_exact": bool(np.array_equal(res_p, opv)),
+                    "values_rel_fro": rel_fro(res_v, ov), "prec_rel_fro": rel_fro(res_p, opv)}
+        del ov, opv
+    elif prec_mode != minifem.PREC_FP64:
+        # against one FP64 step of the same kernel (bit-identical to the oracle's COLORING
+        # order, a property the GPU tests hold at this size)
+        step(minifem.PREC_FP64)
+        stream.sync()
+        v64, p64 = dm.values.numpy(), dm.prec.numpy()
+        accuracy = {"vs": "FP64 step of the same schedule (bit-identical to the oracle)",
+                    "values_rel_fro": rel_fro(res_v, v64), "prec_rel_fro": rel_fro(res_p, p64)}
+        del v64, p64
+    if accuracy is not None:
+        log(f"accuracy: {accuracy}")
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "CSR blocks/s", "n_gpus": 1,
@@ -243,14 +264,26 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_baseline_coloring_omp": cpu_omp,
+        "accuracy": accuracy,
     }
     print(json.dumps(out), flush=True)
+
+
+def rel_fro(a, b):
+    """Relative Frobenius error of a against b over b's finite entries."""
+    import numpy as np
+    m = np.isfinite(b)
+    n = float(np.linalg.norm(b[m]))
+    d = float(np.linalg.norm((a - b)[m]))
+    return d / n if n > 0 else d
 
 
 def bench_batched(args):
     """64 independent meshes assembled by one replay of the captured batch graph, in the
     per-mesh form (one launch per mesh on 8 side streams) and the union form (one
     persistent launch per operator over the disjoint union of its meshes)."""
+    import numpy as np
+
     import minifem
     import minifem.gpu as gpu
     from minifem.batched import Batch, batch_meshes, tune_batch
@@ -262,6 +295,7 @@ def bench_batched(args):
     out = {"metric": "batched CSR blocks assembled/s (64 meshes, one graph launch)",
            "unit": "CSR blocks/s", "n_gpus": 1, "data": "synthetic"}
     B = sum(algorithmic_bytes(m.nbElem, m.nbNodes, m.nnz, m.operatorID) for m in meshes)
+    fp64_results = {}
     for form, union in (("per_mesh", False), ("union", True)):
         best, res = tune_batch(meshes, log=log, union=union)
         out[form] = {"schedule": best}
@@ -273,11 +307,22 @@ def bench_batched(args):
                                "ms_per_batch_eager": round(ms_eager, 4),
                                "launches": b.launches(),
                                "value": round(b.blocks() / (ms_graph * 1e-3), 1)}
+            got = b.results()
+            if name == "fp64":
+                fp64_results[form] = got
+            else:  # the error of every mixed result against the FP64 one, worst mesh
+                ref = fp64_results[form]
+                out[form][name]["max_rel_err_vs_fp64"] = {
+                    "values": max(rel_fro(v, r[0]) for (v, _), r in zip(got, ref)),
+                    "prec": max(rel_fro(p, r[1]) for (_, p), r in zip(got, ref))}
             log(f"{form} {name}: {out[form][name]}")
-            del b
+            del b, got
         gbs = B / (out[form]["fp64"]["ms_per_batch"] * 1e-3) / 1e9
         out[form]["roofline_fp64"] = {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                                       "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    out["union_equals_per_mesh_fp64"] = all(
+        np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        for a, b in zip(fp64_results["per_mesh"], fp64_results["union"]))
     out["union_vs_per_mesh_fp64"] = round(out["union"]["fp64"]["value"] /
                                           out["per_mesh"]["fp64"]["value"], 3)
     out["value"] = max(out["union"]["fp64"]["value"], out["per_mesh"]["fp64"]["value"])
@@ -303,6 +348,8 @@ def bench_serve(args):
     out = {"metric": "served assembly requests: p50/p99 latency and throughput vs plain FP64",
            "unit": "ms", "n_gpus": 1, "data": "synthetic", "requests": len(reqs),
            "rate_hz": args.rate, "hbm_budget_bytes": budget, "higher_is_better": False}
+    sample = reqs[:8]  # requests whose results are checked against plain FP64
+    ref = {}
     for name, prec, slots in (("plain_fp64_one_at_a_time", minifem.PREC_FP64, 1),
                               ("batched_fp64", minifem.PREC_FP64, 4),
                               ("batched_mixed", minifem.PREC_MIXED, 4)):
@@ -310,8 +357,16 @@ def bench_serve(args):
         srv.stage(reqs)
         st = srv.run(reqs)
         out[name] = {k: round(v, 4) for k, v in st.items()}
-        log(f"{name}: {st}")
-        del srv
+        got = {r.rid: srv.run_alone(r) for r in sample}
+        if not ref:
+            ref = got
+        else:
+            out[name]["sample_max_rel_err_vs_plain_fp64"] = {
+                "requests": len(sample),
+                "values": max(rel_fro(got[k][0], ref[k][0]) for k in got),
+                "prec": max(rel_fro(got[k][1], ref[k][1]) for k in got)}
+        log(f"{name}: {out[name]}")
+        del srv, got
     out["throughput_vs_plain"] = {
         k: round(out[k]["requests_per_s"] / out["plain_fp64_one_at_a_time"]["requests_per_s"], 3)
         for k in ("batched_fp64", "batched_mixed")}
