@@ -70,6 +70,8 @@ class UnionMesh:
         coef = (np.concatenate([m.coef_original for m in self.parts])
                 if all(have_coef) else None)
         cap = sum(m.nnz for m in self.parts)
+        if cap > 2**31 - 1:
+            raise ValueError("UnionMesh: too many CSR blocks for 32-bit row pointers")
         self.mesh = Mesh(coord, e2n, code, cap, op, coef=coef)
         if self.mesh.nnz != cap:
             raise AssertionError("UnionMesh: CSR size differs from the sum of the parts")

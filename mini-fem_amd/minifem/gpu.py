@@ -338,7 +338,10 @@ class Plan:
         pc = None
         if use_coord:
             pc = mesh.coord if partition_coord is None else np.ascontiguousarray(
-                partition_coord, np.float64)
+                partition_coord, np.float64).ravel()
+            if pc.size != 3 * mesh.nbNodes:
+                raise ValueError(f"partition_coord has {pc.size} values, the mesh needs "
+                                 f"{3 * mesh.nbNodes} (x, y, z per node)")
         p = ctypes.c_void_p()
         check(self._lib.mf_plan_create(
             ctypes.byref(p), mesh.operatorID, mesh.nbElem, mesh.nbNodes, mesh.nnz,

@@ -226,7 +226,10 @@ def plan_host_stats(mesh, tile_lds_bytes=0, block_threads=0, max_tile_rows=0, us
     from ._lib import PlanParams
     params = PlanParams(int(tile_lds_bytes), int(block_threads), int(max_tile_rows))
     out = np.zeros(8, np.int64)
-    pc = mesh.coord if partition_coord is None else _c(partition_coord, np.float64)
+    pc = mesh.coord if partition_coord is None else _c(partition_coord, np.float64).ravel()
+    if pc.size != 3 * mesh.nbNodes:
+        raise ValueError(f"partition_coord has {pc.size} values, the mesh needs "
+                         f"{3 * mesh.nbNodes} (x, y, z per node)")
     check(lib().mf_plan_host_stats(mesh.operatorID, mesh.nbElem, mesh.nbNodes, mesh.nnz,
                                    _ptr(mesh.elemToNode), _ptr(mesh.row), _ptr(mesh.col),
                                    _ptr(pc) if use_coord else None,

@@ -64,6 +64,20 @@ def test_union_plan_validates(mf, op):
     assert s["tile_elems"] >= u.mesh.nbElem
 
 
+def test_union_of_one_mesh_is_the_mesh(mf):
+    p = parts(1, count=2)[0]
+    u = UnionMesh([p])
+    for a in ("row", "col", "elemToNode", "checkBounds", "colorToElem", "coef"):
+        assert np.array_equal(getattr(u.mesh, a), getattr(p, a)), a
+    assert u.value_range(0) == (0, p.nnz * 9) and u.prec_range(0) == (0, p.nbNodes * 9)
+
+
+def test_partition_coord_size_is_checked(mf):
+    u = UnionMesh(parts(0, count=2))
+    with pytest.raises(ValueError):
+        plan_host_stats(u.mesh, partition_coord=u.partition_coord[:-3])
+
+
 def test_union_rejects_mixed_operators(mf):
     ms = batch_meshes(count=2, n_lo=60, n_hi=200, seed=77)
     with pytest.raises(ValueError):
