@@ -1,9 +1,15 @@
 // Host construction and validation of the tiled gather schedule (layout in plan.h).
+// The partition and the per-tile emission run on up to 16 host threads (MF_PLAN_THREADS
+// overrides the count); the plan does not depend on the thread count: subtrees of the
+// bisection and chunks of tiles are joined back in order.
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "minifem.h"
@@ -13,6 +19,78 @@ namespace mf {
 
 namespace {
 
+int plan_threads()
+{
+    if (const char *e = std::getenv("MF_PLAN_THREADS")) {
+        const int n = std::atoi(e);
+        if (n >= 1) return std::min(n, 64);
+    }
+    const unsigned h = std::thread::hardware_concurrency();
+    return std::max(1, std::min(16, (int)h));
+}
+
+// Map from non-negative ids to ints for the ids of one tile (a few hundred nodes, up to
+// ~1K elements), open addressing, cleared in O(1) by a generation counter. A map that
+// would pass half full reports it (insert returns -2) instead of degrading.
+class SmallMap {
+public:
+    explicit SmallMap(int log2cap)
+        : shift_(32 - log2cap), mask_((1u << log2cap) - 1), key_(1u << log2cap),
+          val_(1u << log2cap), gen_(1u << log2cap, 0)
+    {
+    }
+    void clear()
+    {
+        n_ = 0;
+        if (++cur_ == 0) {
+            std::fill(gen_.begin(), gen_.end(), 0u);
+            cur_ = 1;
+        }
+    }
+    int size() const { return n_; }
+    // 1: inserted, 0: already present (value kept), -2: full
+    int insert(int k, int v)
+    {
+        uint32_t h = slot(k);
+        while (gen_[h] == cur_) {
+            if (key_[h] == k) return 0;
+            h = (h + 1) & mask_;
+        }
+        if (2 * (n_ + 1) > (int)mask_ + 1) return -2;
+        gen_[h] = cur_;
+        key_[h] = k;
+        val_[h] = v;
+        n_++;
+        return 1;
+    }
+    // the value of k, or -1 when absent
+    int get(int k) const
+    {
+        uint32_t h = slot(k);
+        while (gen_[h] == cur_) {
+            if (key_[h] == k) return val_[h];
+            h = (h + 1) & mask_;
+        }
+        return -1;
+    }
+
+private:
+    uint32_t slot(int k) const { return ((uint32_t)k * 2654435761u) >> shift_ & mask_; }
+    int shift_;
+    uint32_t mask_;
+    std::vector<int> key_, val_;
+    std::vector<uint32_t> gen_;
+    uint32_t cur_ = 1;
+    int n_ = 0;
+};
+
+// Per-thread scratch of the partitioner's fit test.
+struct FitScratch {
+    SmallMap rows{10}, elems{12}, nodes{10};
+};
+
+using Leaf = std::pair<int, int>;  // rows order[first, second)
+
 struct Partitioner {
     int N, nE, max_rows, budget, slot, block;
     bool ela;
@@ -20,67 +98,67 @@ struct Partitioner {
     const std::vector<int> &n2eI, &n2eV;
     const double *coord;
     std::vector<int> order;
-    std::vector<std::pair<int, int>> leaves;
-    std::vector<int> elemStamp, nodeStamp;
-    int stampId = 0;
+    int spawn_depth = 0;
+    std::atomic<int> failed{0};
 
     Partitioner(int N_, int nE_, int mr, int bud, int slot_, int block_, bool ela_,
                 const int *e2n_, const int *row_, const int *col_, const std::vector<int> &I,
                 const std::vector<int> &V, const double *c, std::vector<int> rows)
         : N(N_), nE(nE_), max_rows(mr), budget(bud), slot(slot_), block(block_), ela(ela_),
-          e2n(e2n_), row(row_), col(col_), n2eI(I), n2eV(V), coord(c), order(std::move(rows)),
-          elemStamp(nE_, -1), nodeStamp(N_, -1)
+          e2n(e2n_), row(row_), col(col_), n2eI(I), n2eV(V), coord(c), order(std::move(rows))
     {
     }
 
     // Whether the tile made of rows order[lo, hi) fits the LDS budget and every
-    // per-tile limit of the kernel.
-    bool fits(int lo, int hi)
+    // per-tile limit of the kernel (each limit is checked as soon as it can fail).
+    bool fits(int lo, int hi, FitScratch &s) const
     {
         const int R = hi - lo;
         if (R > max_rows || R > kMaxTileRows) return false;
-        stampId++;
-        for (int i = lo; i < hi; i++) nodeStamp[order[i]] = stampId;
-        int64_t TE = 0, NN = R, inc = 0, tasks = 0, ent = 0;
+        s.rows.clear();
+        for (int i = lo; i < hi; i++) s.rows.insert(order[i], 0);
         // tasks: at most one per off-diagonal block, a pair of the tile's rows once
+        int64_t tasks = 0, ent = 0, inc = 0;
         for (int i = lo; i < hi; i++) {
             const int r = order[i];
             ent += row[r + 1] - row[r];
             for (int q = row[r]; q < row[r + 1]; q++) {
                 const int c = col[q] - 1;
-                if (c != r && (nodeStamp[c] != stampId || c > r)) tasks++;
+                if (c != r && (s.rows.get(c) < 0 || c > r)) tasks++;
             }
-        }
-        for (int i = lo; i < hi; i++) {
-            const int r = order[i];
             inc += (n2eI[r + 1] - n2eI[r] + kIncAlign - 1) / kIncAlign * kIncAlign;
-            for (int j = n2eI[r]; j < n2eI[r + 1]; j++) {
-                const int e = n2eV[j];
-                if (elemStamp[e] == stampId) continue;
-                elemStamp[e] = stampId;
-                TE++;
-                for (int k = 0; k < 4; k++) {
-                    const int v = e2n[(int64_t)e * 4 + k] - 1;
-                    if (nodeStamp[v] != stampId) { nodeStamp[v] = stampId; NN++; }
-                }
-            }
         }
-        if (NN > kMaxTileNodes || TE > max_tile_elems(block)) return false;
         const int64_t inc8 = (inc + 7) / 8 * 8;
         if (inc8 > kMaxTileInc || ent > kMaxLdsBytes) return false;
         if (!tile_blocks_fit(ela, block, diag_blocks(ela, R), (int)((tasks + kLanes - 1) / kLanes)))
             return false;
-        return tile_lds_layout(ela, (int)NN, (int)TE, (int)TE, R, (int)inc8, (int)tasks, (int)ent,
-                               slot).total <= budget;
+        s.elems.clear();
+        s.nodes.clear();
+        for (int i = lo; i < hi; i++) s.nodes.insert(order[i], 0);
+        int TE = 0, NN = R;
+        const int maxTE = max_tile_elems(block);
+        for (int i = lo; i < hi; i++) {
+            const int r = order[i];
+            for (int j = n2eI[r]; j < n2eI[r + 1]; j++) {
+                const int e = n2eV[j];
+                if (s.elems.insert(e, 0) != 1) continue;
+                if (++TE > maxTE) return false;
+                for (int k = 0; k < 4; k++)
+                    if (s.nodes.insert(e2n[(int64_t)e * 4 + k] - 1, 0) == 1 && ++NN > kMaxTileNodes)
+                        return false;
+            }
+        }
+        return tile_lds_layout(ela, NN, TE, TE, R, (int)inc8, (int)tasks, (int)ent, slot).total <=
+               budget;
     }
 
     // Recursive coordinate bisection along the longest axis. A region whose halves do
     // not fit but whose thirds do is cut in three: tiles of a third rather than a quarter
-    // of it, i.e. fewer and larger tiles.
-    void rcb(int lo, int hi)
+    // of it, i.e. fewer and larger tiles. The top levels run their halves on two threads.
+    void rcb(int lo, int hi, FitScratch &s, std::vector<Leaf> &out, int depth)
     {
-        if (hi - lo <= 1 || fits(lo, hi)) {
-            if (hi > lo) leaves.emplace_back(lo, hi);
+        if (hi - lo <= 1 || fits(lo, hi, s)) {
+            if (hi > lo) out.emplace_back(lo, hi);
             return;
         }
         double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
@@ -100,26 +178,51 @@ struct Partitioner {
         };
         const int mid = lo + (hi - lo) / 2;
         std::nth_element(order.begin() + lo, order.begin() + mid, order.begin() + hi, less);
-        if (hi - lo >= 6 && !(fits(lo, mid) && fits(mid, hi))) {
+        // thirds can fit only when the first one has at most max_rows rows
+        if (hi - lo >= 6 && (hi - lo) / 3 <= max_rows && !(fits(lo, mid, s) && fits(mid, hi, s))) {
             const int t1 = lo + (hi - lo) / 3, t2 = lo + 2 * (hi - lo) / 3;
             std::vector<int> saved(order.begin() + lo, order.begin() + hi);
             std::nth_element(order.begin() + lo, order.begin() + t1, order.begin() + hi, less);
             std::nth_element(order.begin() + t1, order.begin() + t2, order.begin() + hi, less);
-            if (fits(lo, t1) && fits(t1, t2) && fits(t2, hi)) {
-                leaves.emplace_back(lo, t1);
-                leaves.emplace_back(t1, t2);
-                leaves.emplace_back(t2, hi);
+            if (fits(lo, t1, s) && fits(t1, t2, s) && fits(t2, hi, s)) {
+                out.emplace_back(lo, t1);
+                out.emplace_back(t1, t2);
+                out.emplace_back(t2, hi);
                 return;
             }
             std::copy(saved.begin(), saved.end(), order.begin() + lo);  // back to the halves
         }
-        rcb(lo, mid);
-        rcb(mid, hi);
+        if (depth < spawn_depth && hi - lo > 4 * max_rows) {
+            std::vector<Leaf> left;
+            bool spawned = false;
+            std::thread t;
+            try {
+                t = std::thread([&] {
+                    try {
+                        FitScratch s2;
+                        rcb(lo, mid, s2, left, depth + 1);
+                    } catch (...) {
+                        failed = 1;
+                    }
+                });
+                spawned = true;
+            } catch (...) {
+            }
+            if (!spawned) rcb(lo, mid, s, left, depth + 1);
+            std::vector<Leaf> right;
+            rcb(mid, hi, s, right, depth + 1);
+            if (spawned) t.join();
+            out.insert(out.end(), left.begin(), left.end());
+            out.insert(out.end(), right.begin(), right.end());
+            return;
+        }
+        rcb(lo, mid, s, out, depth + 1);
+        rcb(mid, hi, s, out, depth + 1);
     }
 
     // Breadth-first order of the node graph (through shared elements), then greedy
     // chunks that fit.
-    void bfs_chunks()
+    void bfs_chunks(std::vector<Leaf> &out)
     {
         std::vector<char> isRow(N, 0), seen(N, 0);
         for (int r : order) isRow[r] = 1;
@@ -142,13 +245,188 @@ struct Partitioner {
             }
         }
         order = q;
+        FitScratch s;
         const int n = (int)order.size();
         int lo = 0;
         while (lo < n) {
             int hi = lo + 1;
-            while (hi < n && fits(lo, hi + 1)) hi++;
-            leaves.emplace_back(lo, hi);
+            while (hi < n && fits(lo, hi + 1, s)) hi++;
+            out.emplace_back(lo, hi);
             lo = hi;
+        }
+    }
+};
+
+// The tiles of one contiguous range of leaves, with offsets into its own arrays.
+struct TileChunk {
+    std::vector<TileHdr> tiles;
+    std::vector<int32_t> tile_elems;
+    std::vector<uint8_t> tile_verts;
+    std::vector<int32_t> tile_nodes;
+    std::vector<RowInfo> tile_rowinfo;
+    std::vector<uint16_t> inc;
+    std::vector<uint32_t> task_rec;
+    int max_node = 0, max_elem = 0, max_rows = 0;
+    int status = MF_OK;
+};
+
+struct Emitter {
+    bool ela;
+    int N;
+    const int *e2n, *row, *col, *checkBounds;
+    const std::vector<int> &n2eI, &n2eV, &order, &nodeTile;
+    const std::vector<Leaf> &leaves;
+
+    int64_t find_pos(int c, int a) const  // CSR position of block (c, a)
+    {
+        for (int q = row[c]; q < row[c + 1]; q++)
+            if (col[q] == a + 1) return q;
+        return -1;
+    }
+
+    // Tile ti appended to o.
+    int emit(int ti, TileChunk &o, SmallMap &nodeLocal, SmallMap &elemLocal,
+             std::vector<int> &rows, std::vector<int> &tel) const
+    {
+        const Leaf &lf = leaves[ti];
+        rows.assign(order.begin() + lf.first, order.begin() + lf.second);
+        std::sort(rows.begin(), rows.end());
+        const int R = (int)rows.size();
+        if (R > kMaxTileRows) return MF_ERR_ARG;
+        TileHdr h{};
+        h.node_start = (int)o.tile_nodes.size();
+        h.n_rows = R;
+        h.rinfo_start = (int)o.tile_rowinfo.size();
+        nodeLocal.clear();
+        for (int i = 0; i < R; i++) {
+            nodeLocal.insert(rows[i], i);
+            o.tile_nodes.push_back(rows[i]);
+        }
+        tel.clear();
+        elemLocal.clear();
+        for (int r : rows)
+            for (int j = n2eI[r]; j < n2eI[r + 1]; j++) {
+                const int ins = elemLocal.insert(n2eV[j], 0);
+                if (ins == -2) return MF_ERR_ARG;
+                if (ins == 1) tel.push_back(n2eV[j]);
+            }
+        std::sort(tel.begin(), tel.end());
+        const int TE = (int)tel.size();
+        if (TE > kMaxTileElems) return MF_ERR_ARG;
+        elemLocal.clear();
+        for (int t = 0; t < TE; t++) elemLocal.insert(tel[t], t);
+        for (int t = 0; t < TE; t++)
+            for (int k = 0; k < 4; k++) {
+                const int v = e2n[(int64_t)tel[t] * 4 + k] - 1;
+                const int ins = nodeLocal.insert(v, (int)o.tile_nodes.size() - h.node_start);
+                if (ins == -2) return MF_ERR_ARG;
+                if (ins == 1) o.tile_nodes.push_back(v);
+            }
+        h.n_node = (int)o.tile_nodes.size() - h.node_start;
+        if (h.n_node > kMaxTileNodes) return MF_ERR_ARG;
+        h.elem_start = (int)o.tile_elems.size();
+        h.n_elem = TE;
+        for (int t = 0; t < TE; t++) {
+            o.tile_elems.push_back(tel[t]);
+            for (int k = 0; k < 4; k++)
+                o.tile_verts.push_back((uint8_t)nodeLocal.get(e2n[(int64_t)tel[t] * 4 + k] - 1));
+        }
+
+        // tile entries: the rows' CSR segments in tile row order
+        std::vector<int> ent((size_t)R + 1, 0);
+        for (int ra = 0; ra < R; ra++) ent[ra + 1] = ent[ra] + row[rows[ra] + 1] - row[rows[ra]];
+        h.n_ent = ent[R];
+
+        // incidence lists (ascending element = ascending tile element) and entries
+        h.inc_start = (int)o.inc.size();
+        h.task_start = (int)(o.task_rec.size() / 4);
+        int incOff = 0;
+        for (int ra = 0; ra < R; ra++) {
+            const int a = rows[ra], len = row[a + 1] - row[a];
+            const int val = n2eI[a + 1] - n2eI[a];
+            int doff = -1;
+            for (int q = 0; q < len; q++)
+                if (col[row[a] + q] - 1 == a) { doff = q; break; }
+            if (doff < 0 || doff > 0xFF) return MF_ERR_ARG;
+            int mask = 0;
+            if (ela && checkBounds)
+                for (int k = 0; k < 3; k++)
+                    if (checkBounds[(int64_t)k * N + a] != 0) mask |= 1 << k;
+            o.tile_rowinfo.push_back(RowInfo{row[a], len | doff << 16 | mask << 24, incOff | val << 16, ent[ra]});
+            for (int j = n2eI[a]; j < n2eI[a + 1]; j++) {
+                const int e = n2eV[j];
+                int va = 0;
+                for (int k = 0; k < 4; k++)
+                    if (e2n[(int64_t)e * 4 + k] - 1 == a) va = k;
+                o.inc.push_back((uint16_t)(elemLocal.get(e) << 2 | va));
+            }
+            for (int q = 0; q < len; q++) {
+                const int c = col[row[a] + q] - 1;
+                if (c < 0 || c >= N || nodeLocal.get(c) < 0 || q > 0xFF) return MF_ERR_ARG;
+                if (c == a) continue;
+                // pair (a,c)/(c,a) inside one tile: computed once, by the row with the
+                // smaller node id, the transpose written into the tile's output image;
+                // a column of another tile: that tile computes (c,a) itself
+                int oc = 0;
+                if (nodeTile[c] == ti) {
+                    if (c < a) continue;
+                    const int64_t pc = find_pos(c, a);
+                    if (pc < 0) return MF_ERR_ARG;  // the pattern is not symmetric
+                    oc = (int)(pc - row[c]);
+                    if (oc > 0xFF) return MF_ERR_ARG;
+                }
+                // the elements holding a and c, ascending, with a's and c's slots
+                uint64_t m = 0;
+                uint16_t list[kListLen];
+                int cnt = 0;
+                for (int j = n2eI[a]; j < n2eI[a + 1]; j++) {
+                    const int *v = e2n + (int64_t)n2eV[j] * 4;
+                    int va = -1, vc = -1;
+                    for (int k = 0; k < 4; k++) {
+                        if (v[k] - 1 == a) va = k;
+                        if (v[k] - 1 == c) vc = k;
+                    }
+                    if (vc < 0) continue;
+                    m |= 1ull << (j - n2eI[a]);
+                    if (cnt < kListLen) list[cnt] = (uint16_t)(elemLocal.get(n2eV[j]) << 4 | va << 2 | vc);
+                    cnt++;
+                }
+                if (cnt < 1) return MF_ERR_ARG;
+                if (cnt <= kListLen) {
+                    for (int z = cnt; z < kListLen; z++) list[z] = (uint16_t)(TE << 4);
+                    for (int w = 0; w < 3; w++)
+                        o.task_rec.push_back((uint32_t)list[2 * w] | (uint32_t)list[2 * w + 1] << 16);
+                } else {
+                    o.task_rec.push_back(kMaskRecord);
+                    o.task_rec.push_back((uint32_t)m);
+                    o.task_rec.push_back((uint32_t)(m >> 32));
+                }
+                o.task_rec.push_back((uint32_t)(ra | q << 8 | nodeLocal.get(c) << 16 | oc << 24));
+            }
+            while (o.inc.size() % kIncAlign) o.inc.push_back(0);
+            incOff = (int)o.inc.size() - h.inc_start;
+        }
+        while (o.inc.size() % 8) o.inc.push_back(0);
+        h.n_inc = (int)o.inc.size() - h.inc_start;
+        h.n_task = (int)(o.task_rec.size() / 4) - h.task_start;
+        o.tiles.push_back(h);
+        o.max_node = std::max(o.max_node, h.n_node);
+        o.max_elem = std::max(o.max_elem, TE);
+        o.max_rows = std::max(o.max_rows, R);
+        if (o.inc.size() > (size_t)INT32_MAX - 100000 || o.task_rec.size() > (size_t)INT32_MAX / 2)
+            return MF_ERR_NOMEM;
+        return MF_OK;
+    }
+
+    void emit_range(int t0, int t1, TileChunk &o) const
+    {
+        try {
+            SmallMap nodeLocal(10), elemLocal(12);
+            std::vector<int> rows, tel;
+            for (int ti = t0; ti < t1 && o.status == MF_OK; ti++)
+                o.status = emit(ti, o, nodeLocal, elemLocal, rows, tel);
+        } catch (...) {
+            o.status = MF_ERR_NOMEM;
         }
     }
 };
@@ -197,171 +475,102 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
     for (int i = 0; i < N; i++)
         if ((n2eI[i + 1] > n2eI[i]) != (row[i + 1] > row[i])) return MF_ERR_ARG;
 
+    const int threads = plan_threads();
     Partitioner part(N, nE, max_rows, lds_budget, slot_bytes, block, ela, e2n, row, col, n2eI,
                      n2eV, coord, rowsAll);
-    if (coord) part.rcb(0, (int)rowsAll.size());
-    else part.bfs_chunks();
+    while ((1 << part.spawn_depth) < threads) part.spawn_depth++;
+    std::vector<Leaf> leaves;
+    if (coord) {
+        FitScratch s;
+        part.rcb(0, (int)rowsAll.size(), s, leaves, 0);
+    } else {
+        part.bfs_chunks(leaves);
+    }
+    if (part.failed) return MF_ERR_NOMEM;
 
     // tile of every row node (the kernel runs tile i on XCD i / ceil(T / 8))
-    const int T = (int)part.leaves.size();
+    const int T = (int)leaves.size();
     std::vector<int> nodeTile((size_t)N, -1);
     for (int i = 0; i < T; i++)
-        for (int q = part.leaves[i].first; q < part.leaves[i].second; q++) nodeTile[part.order[q]] = i;
-    auto find_pos = [&](int c, int a) -> int64_t {  // CSR position of block (c, a)
-        for (int q = row[c]; q < row[c + 1]; q++)
-            if (col[q] == a + 1) return q;
-        return -1;
-    };
+        for (int q = leaves[i].first; q < leaves[i].second; q++) nodeTile[part.order[q]] = i;
 
-    std::vector<int> nodeLocal((size_t)N, -1), elemLocal((size_t)nE, -1);
-    std::vector<int> rows, tel;
-    for (int ti = 0; ti < T; ti++) {
-        const auto &lf = part.leaves[ti];
-        rows.assign(part.order.begin() + lf.first, part.order.begin() + lf.second);
-        std::sort(rows.begin(), rows.end());
-        const int R = (int)rows.size();
-        if (R > kMaxTileRows) return MF_ERR_ARG;
-        TileHdr h{};
-        h.node_start = (int)p.tile_nodes.size();
-        h.n_rows = R;
-        h.rinfo_start = (int)p.tile_rowinfo.size();
-        for (int i = 0; i < R; i++) {
-            nodeLocal[rows[i]] = i;
-            p.tile_nodes.push_back(rows[i]);
+    // the tiles in contiguous chunks, one per thread, then joined in order
+    const Emitter em{ela, N, e2n, row, col, checkBounds, n2eI, n2eV, part.order, nodeTile, leaves};
+    const int nc = std::max(1, std::min(threads, T / 64));
+    std::vector<TileChunk> chunks((size_t)nc);
+    std::vector<std::thread> pool;
+    for (int c = 1; c < nc; c++) {
+        try {
+            pool.emplace_back([&, c] { em.emit_range((int)((int64_t)T * c / nc),
+                                                     (int)((int64_t)T * (c + 1) / nc), chunks[c]); });
+        } catch (...) {  // no thread: run that chunk here
+            em.emit_range((int)((int64_t)T * c / nc), (int)((int64_t)T * (c + 1) / nc), chunks[c]);
         }
-        tel.clear();
-        for (int r : rows)
-            for (int j = n2eI[r]; j < n2eI[r + 1]; j++) {
-                const int e = n2eV[j];
-                if (elemLocal[e] == -1) { elemLocal[e] = -2; tel.push_back(e); }
-            }
-        std::sort(tel.begin(), tel.end());
-        const int TE = (int)tel.size();
-        if (TE > kMaxTileElems) return MF_ERR_ARG;
-        for (int t = 0; t < TE; t++) elemLocal[tel[t]] = t;
-        for (int t = 0; t < TE; t++)
-            for (int k = 0; k < 4; k++) {
-                const int v = e2n[(int64_t)tel[t] * 4 + k] - 1;
-                if (nodeLocal[v] == -1) {
-                    nodeLocal[v] = (int)p.tile_nodes.size() - h.node_start;
-                    p.tile_nodes.push_back(v);
-                }
-            }
-        h.n_node = (int)p.tile_nodes.size() - h.node_start;
-        if (h.n_node > kMaxTileNodes) return MF_ERR_ARG;
-        h.elem_start = (int)p.tile_elems.size();
-        h.n_elem = TE;
-        for (int t = 0; t < TE; t++) {
-            p.tile_elems.push_back(tel[t]);
-            for (int k = 0; k < 4; k++)
-                p.tile_verts.push_back((uint8_t)nodeLocal[e2n[(int64_t)tel[t] * 4 + k] - 1]);
+    }
+    em.emit_range(0, (int)((int64_t)T / nc), chunks[0]);
+    for (auto &t : pool) t.join();
+    size_t nTiles = 0, nElems = 0, nNodes = 0, nRinfo = 0, nInc = 0, nTask = 0;
+    for (const TileChunk &c : chunks) {
+        if (c.status != MF_OK) return c.status;
+        nTiles += c.tiles.size();
+        nElems += c.tile_elems.size();
+        nNodes += c.tile_nodes.size();
+        nRinfo += c.tile_rowinfo.size();
+        nInc += c.inc.size();
+        nTask += c.task_rec.size();
+    }
+    if (nInc > (size_t)INT32_MAX - 100000 || nTask > (size_t)INT32_MAX / 2 ||
+        nNodes > (size_t)INT32_MAX || nElems > (size_t)INT32_MAX / 4)
+        return MF_ERR_NOMEM;
+    p.tiles.reserve(nTiles);
+    p.tile_elems.reserve(nElems);
+    p.tile_verts.reserve(nElems * 4);
+    p.tile_nodes.reserve(nNodes);
+    p.tile_rowinfo.reserve(nRinfo);
+    p.inc.reserve(nInc);
+    p.task_rec.reserve(nTask);
+    for (TileChunk &c : chunks) {
+        const int oN = (int)p.tile_nodes.size(), oE = (int)p.tile_elems.size(),
+                  oR = (int)p.tile_rowinfo.size(), oI = (int)p.inc.size(),
+                  oT = (int)(p.task_rec.size() / 4);
+        for (TileHdr h : c.tiles) {
+            h.node_start += oN;
+            h.elem_start += oE;
+            h.rinfo_start += oR;
+            h.inc_start += oI;
+            h.task_start += oT;
+            p.tiles.push_back(h);
         }
-
-        // tile entries: the rows' CSR segments in tile row order
-        std::vector<int> ent((size_t)R + 1, 0);
-        for (int ra = 0; ra < R; ra++) ent[ra + 1] = ent[ra] + row[rows[ra] + 1] - row[rows[ra]];
-        h.n_ent = ent[R];
-
-        // incidence lists (ascending element = ascending tile element) and entries
-        h.inc_start = (int)p.inc.size();
-        h.task_start = (int)(p.task_rec.size() / 4);
-        int incOff = 0;
-        for (int ra = 0; ra < R; ra++) {
-            const int a = rows[ra], len = row[a + 1] - row[a];
-            const int val = n2eI[a + 1] - n2eI[a];
-            int doff = -1;
-            for (int o = 0; o < len; o++)
-                if (col[row[a] + o] - 1 == a) { doff = o; break; }
-            if (doff < 0 || doff > 0xFF) return MF_ERR_ARG;
-            int mask = 0;
-            if (ela && checkBounds)
-                for (int k = 0; k < 3; k++)
-                    if (checkBounds[(int64_t)k * N + a] != 0) mask |= 1 << k;
-            p.tile_rowinfo.push_back(RowInfo{row[a], len | doff << 16 | mask << 24, incOff | val << 16, ent[ra]});
-            for (int j = n2eI[a]; j < n2eI[a + 1]; j++) {
-                const int e = n2eV[j];
-                int va = 0;
-                for (int k = 0; k < 4; k++)
-                    if (e2n[(int64_t)e * 4 + k] - 1 == a) va = k;
-                p.inc.push_back((uint16_t)(elemLocal[e] << 2 | va));
-            }
-            for (int o = 0; o < len; o++) {
-                const int c = col[row[a] + o] - 1;
-                if (c < 0 || c >= N || nodeLocal[c] < 0 || o > 0xFF) return MF_ERR_ARG;
-                if (c == a) continue;
-                // pair (a,c)/(c,a) inside one tile: computed once, by the row with the
-                // smaller node id, the transpose written into the tile's output image;
-                // a column of another tile: that tile computes (c,a) itself
-                int oc = 0;
-                if (nodeTile[c] == ti) {
-                    if (c < a) continue;
-                    const int64_t q = find_pos(c, a);
-                    if (q < 0) return MF_ERR_ARG;  // the pattern is not symmetric
-                    oc = (int)(q - row[c]);
-                    if (oc > 0xFF) return MF_ERR_ARG;
-                }
-                // the elements holding a and c, ascending, with a's and c's slots
-                uint64_t m = 0;
-                uint16_t list[kListLen];
-                int cnt = 0;
-                for (int j = n2eI[a]; j < n2eI[a + 1]; j++) {
-                    const int *v = e2n + (int64_t)n2eV[j] * 4;
-                    int va = -1, vc = -1;
-                    for (int k = 0; k < 4; k++) {
-                        if (v[k] - 1 == a) va = k;
-                        if (v[k] - 1 == c) vc = k;
-                    }
-                    if (vc < 0) continue;
-                    m |= 1ull << (j - n2eI[a]);
-                    if (cnt < kListLen) list[cnt] = (uint16_t)(elemLocal[n2eV[j]] << 4 | va << 2 | vc);
-                    cnt++;
-                }
-                if (cnt < 1) return MF_ERR_ARG;
-                if (cnt <= kListLen) {
-                    for (int q = cnt; q < kListLen; q++) list[q] = (uint16_t)(TE << 4);
-                    for (int w = 0; w < 3; w++)
-                        p.task_rec.push_back((uint32_t)list[2 * w] | (uint32_t)list[2 * w + 1] << 16);
-                } else {
-                    p.task_rec.push_back(kMaskRecord);
-                    p.task_rec.push_back((uint32_t)m);
-                    p.task_rec.push_back((uint32_t)(m >> 32));
-                }
-                p.task_rec.push_back((uint32_t)(ra | o << 8 | nodeLocal[c] << 16 | oc << 24));
-            }
-            while (p.inc.size() % kIncAlign) p.inc.push_back(0);
-            incOff = (int)p.inc.size() - h.inc_start;
-        }
-        while (p.inc.size() % 8) p.inc.push_back(0);
-        h.n_inc = (int)p.inc.size() - h.inc_start;
-        h.n_task = (int)(p.task_rec.size() / 4) - h.task_start;
-        p.tiles.push_back(h);
-        p.max_node = std::max(p.max_node, h.n_node);
-        p.max_elem = std::max(p.max_elem, TE);
-        p.max_rows = std::max(p.max_rows, R);
-        for (int i = h.node_start; i < (int)p.tile_nodes.size(); i++) nodeLocal[p.tile_nodes[i]] = -1;
-        for (int e : tel) elemLocal[e] = -1;
-        if (p.inc.size() > (size_t)INT32_MAX - 100000 || p.task_rec.size() > (size_t)INT32_MAX / 2)
-            return MF_ERR_NOMEM;
+        p.tile_elems.insert(p.tile_elems.end(), c.tile_elems.begin(), c.tile_elems.end());
+        p.tile_verts.insert(p.tile_verts.end(), c.tile_verts.begin(), c.tile_verts.end());
+        p.tile_nodes.insert(p.tile_nodes.end(), c.tile_nodes.begin(), c.tile_nodes.end());
+        p.tile_rowinfo.insert(p.tile_rowinfo.end(), c.tile_rowinfo.begin(), c.tile_rowinfo.end());
+        p.inc.insert(p.inc.end(), c.inc.begin(), c.inc.end());
+        p.task_rec.insert(p.task_rec.end(), c.task_rec.begin(), c.task_rec.end());
+        p.max_node = std::max(p.max_node, c.max_node);
+        p.max_elem = std::max(p.max_elem, c.max_elem);
+        p.max_rows = std::max(p.max_rows, c.max_rows);
+        c = TileChunk{};  // release as we go
     }
     return MF_OK;
 }
 
-int validate_plan(const PlanHost &p, const int *row, const int *col, const int *e2n,
-                  int slot_bytes)
-{
-    const bool ela = p.op == MF_OP_ELA;
-    if (p.tile_verts.size() != p.tile_elems.size() * 4) return MF_ERR_STATE;
-    if (p.task_rec.size() % 4) return MF_ERR_STATE;
-    const size_t nTasks = p.task_rec.size() / 4;
-    std::vector<uint8_t> hit((size_t)p.nnz, 0), rowSeen((size_t)p.N, 0);
-    for (int i : p.isolated) {
-        if (i < 0 || i >= p.N || row[i + 1] != row[i] || rowSeen[i]) return MF_ERR_STATE;
-        rowSeen[i] = 1;
-    }
-    std::vector<int> nodeStamp((size_t)p.N, -1), localOf((size_t)p.N, -1), cnt(kMaxTileNodes);
-    std::vector<uint64_t> holds(kMaxTileNodes, 0);
-    int tileNo = 0;
-    for (const TileHdr &h : p.tiles) {
+namespace {
+
+// Claims a CSR entry or a row for one writer; false when it was claimed before.
+inline bool claim(uint8_t *flag) { return __atomic_fetch_add(flag, 1, __ATOMIC_RELAXED) == 0; }
+
+struct TileValidator {
+    const PlanHost &p;
+    const int *row, *col, *e2n;
+    int slot_bytes;
+    bool ela;
+    size_t nTasks;
+    uint8_t *hit, *rowSeen;  // shared by all tiles, claimed atomically
+
+    int check(const TileHdr &h, SmallMap &localOf, std::vector<int> &cnt,
+              std::vector<uint64_t> &holds) const
+    {
         const int R = h.n_rows;
         if (R < 1 || R > kMaxTileRows || h.n_node < R || h.n_node > kMaxTileNodes ||
             h.n_elem < 0 || h.n_elem > max_tile_elems(p.block))
@@ -393,16 +602,11 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
         }
         const int32_t *nodes = p.tile_nodes.data() + h.node_start;
         const uint8_t *verts = p.tile_verts.data() + (size_t)h.elem_start * 4;
-        tileNo++;
+        localOf.clear();
         for (int i = 0; i < h.n_node; i++) {
             const int n = nodes[i];
-            if (n < 0 || n >= p.N || nodeStamp[n] == tileNo) return MF_ERR_STATE;
-            nodeStamp[n] = tileNo;
-            localOf[n] = i;
-            if (i < R) {
-                if (rowSeen[n]) return MF_ERR_STATE;
-                rowSeen[n] = 1;
-            }
+            if (n < 0 || n >= p.N || localOf.insert(n, i) != 1) return MF_ERR_STATE;
+            if (i < R && !claim(rowSeen + n)) return MF_ERR_STATE;
         }
         for (int t = 0; t < h.n_elem; t++) {
             const int e = p.tile_elems[h.elem_start + t];
@@ -431,7 +635,7 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
                 incOff + val > h.n_inc)
                 return MF_ERR_STATE;
             incNext += (val + kIncAlign - 1) / kIncAlign * kIncAlign;
-            if (hit[(size_t)row[a] + doff]++) return MF_ERR_STATE;  // the diagonal block
+            if (!claim(hit + row[a] + doff)) return MF_ERR_STATE;  // the diagonal block
             // the incidence list: every element of the tile holding a, ascending
             int last = -1;
             for (int i = 0; i < val; i++) {
@@ -448,8 +652,8 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
                 const int oa = (rec[3] >> 8) & 0xFF, cl = (rec[3] >> 16) & 0xFF, oc = rec[3] >> 24;
                 if (oa >= len) return MF_ERR_STATE;
                 const int c = col[row[a] + oa] - 1;
-                if (c == a || hit[(size_t)row[a] + oa]++) return MF_ERR_STATE;
-                const int ec = nodeStamp[c] == tileNo ? localOf[c] : -1;
+                if (c == a || !claim(hit + row[a] + oa)) return MF_ERR_STATE;
+                const int ec = localOf.get(c);
                 const int cnt = ec < 0 ? 0 : __builtin_popcountll(holds[ec]);
                 if (ec < 0 || cl != ec || cnt < 1) return MF_ERR_STATE;
                 if (cnt <= kListLen) {
@@ -476,7 +680,7 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
                 }
                 if (ec < R) {  // c is a row of this tile: the task also writes (c, a)
                     if (oc >= (p.tile_rowinfo[h.rinfo_start + ec].len_diag & 0xFFFF) ||
-                        col[row[c] + oc] != a + 1 || hit[(size_t)row[c] + oc]++)
+                        col[row[c] + oc] != a + 1 || !claim(hit + row[c] + oc))
                         return MF_ERR_STATE;
                 } else if (oc != 0) {
                     return MF_ERR_STATE;
@@ -488,9 +692,61 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
             }
         }
         if (tk != tkEnd || incNext > h.n_inc || h.n_inc - incNext >= 8) return MF_ERR_STATE;
+        return MF_OK;
     }
+};
+
+}  // namespace
+
+int validate_plan(const PlanHost &p, const int *row, const int *col, const int *e2n,
+                  int slot_bytes)
+{
+    if (p.tile_verts.size() != p.tile_elems.size() * 4) return MF_ERR_STATE;
+    if (p.task_rec.size() % 4) return MF_ERR_STATE;
+    std::vector<uint8_t> hit((size_t)p.nnz, 0), rowSeen((size_t)p.N, 0);
+    for (int i : p.isolated) {
+        if (i < 0 || i >= p.N || row[i + 1] != row[i] || rowSeen[i]) return MF_ERR_STATE;
+        rowSeen[i] = 1;
+    }
+    const TileValidator tv{p, row, col, e2n, slot_bytes, p.op == MF_OP_ELA, p.task_rec.size() / 4,
+                           hit.data(), rowSeen.data()};
+    // the tiles in contiguous chunks, one per thread; the first failure in any stops all
+    const int T = (int)p.tiles.size();
+    const int nc = std::max(1, std::min(plan_threads(), T / 64));
+    std::vector<int> status((size_t)nc, MF_OK);
+    std::atomic<int> stop{0};
+    auto run = [&](int c) {
+        try {
+            SmallMap localOf(10);
+            std::vector<int> cnt(kMaxTileNodes);
+            std::vector<uint64_t> holds(kMaxTileNodes, 0);
+            const int t1 = (int)((int64_t)T * (c + 1) / nc);
+            for (int t = (int)((int64_t)T * c / nc); t < t1 && !stop; t++) {
+                const int st = tv.check(p.tiles[t], localOf, cnt, holds);
+                if (st != MF_OK) {
+                    status[c] = st;
+                    stop = 1;
+                }
+            }
+        } catch (...) {
+            status[c] = MF_ERR_NOMEM;
+            stop = 1;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int c = 1; c < nc; c++) {
+        try {
+            pool.emplace_back(run, c);
+        } catch (...) {
+            run(c);
+        }
+    }
+    run(0);
+    for (auto &t : pool) t.join();
+    for (int st : status)
+        if (st != MF_OK) return st;
     for (int i = 0; i < p.N; i++)
-        if (!rowSeen[i]) return MF_ERR_STATE;
+        if (rowSeen[i] != 1) return MF_ERR_STATE;
     for (int64_t q = 0; q < p.nnz; q++)
         if (hit[q] != 1) return MF_ERR_STATE;
     return MF_OK;

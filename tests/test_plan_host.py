@@ -87,3 +87,17 @@ def test_plan_accepts_valence_between_32_and_64(mf):
     mesh = mf.Mesh(coord.ravel(), e2n, code, nnz, 0)
     s = plan_host_stats(mesh)
     assert s["tasks"] <= mesh.nnz - mesh.nbNodes
+
+
+@pytest.mark.parametrize("use_coord", [True, False])
+def test_plan_independent_of_host_threads(mf, monkeypatch, use_coord):
+    """The partition and the tile emission run on several host threads; the plan is the
+    same with one (MF_PLAN_THREADS overrides the count read at each build)."""
+    coord, e2n, code, nnz = mf.generate_cube(30, 0.2, 3, 209)
+    mesh = mf.Mesh(coord, e2n, code, nnz, 1)
+    got = []
+    for threads in ("1", "4"):
+        monkeypatch.setenv("MF_PLAN_THREADS", threads)
+        got.append(plan_host_stats(mesh, use_coord=use_coord))
+    assert got[0]["tiles"] >= 128  # enough tiles for several emission chunks
+    assert got[0] == got[1]
