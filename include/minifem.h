@@ -264,7 +264,9 @@ typedef struct mf_plan_params {
  * checkBounds[3N] (may be NULL), coord[3N] (may be NULL; used only to group nodes into
  * tiles by coordinate bisection, else a breadth-first node order is used). The plan
  * copies what it needs to the GPU and checks every index its kernel follows. Limit: at
- * most 64 elements per node (MF_ERR_ARG otherwise; the colored path has none). */
+ * most 64 elements per node (MF_ERR_ARG otherwise; the colored path has none). The
+ * build runs on up to 16 host threads (environment MF_PLAN_THREADS sets the count); the
+ * plan does not depend on the count. */
 int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nbNodes, int nnz,
                    const int *elemToNode, const int *row, const int *col,
                    const int *checkBounds, const double *coord,
