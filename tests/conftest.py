@@ -10,7 +10,7 @@ for p in (os.path.join(ROOT, "mini-fem_amd"), os.path.join(ROOT, "oracle"), ROOT
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU")
     config.addinivalue_line("markers", "slow: larger CPU cases")
 
 
