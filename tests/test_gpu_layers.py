@@ -77,6 +77,26 @@ def test_batched_union_equals_per_mesh(mf, gpu, precision):
         del u
 
 
+@pytest.mark.parametrize("op", [0, 1])
+def test_batched_union_with_isolated_node(mf, gpu, op):
+    """A union whose part has an unused node (an empty CSR row, prec of a node without
+    elements) equals the per-mesh form bit for bit."""
+    from minifem.batched import Batch
+    print(f"[batched] union with an isolated node, op {op}", flush=True)
+    iso = mf.Mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [5, 5, 5]],
+                           np.float64).ravel(),
+                  np.array([1, 2, 3, 4], np.int32), np.array([0, 0, 0, 0, 53], np.int32), 16, op)
+    meshes = [mf.Mesh.cube(5, 0.2, 11, op), iso, mf.Mesh.cube(7, 0.2, 12, op, shuffle_seed=5)]
+    ref = Batch(meshes, branches=2)
+    ref.run_eager()
+    want = ref.results()
+    u = Batch(meshes, union=True).capture()
+    u.launch()
+    u.stream.sync()
+    for i, ((v0, p0), (v1, p1)) in enumerate(zip(want, u.results())):
+        assert np.array_equal(v0, v1) and np.array_equal(p0, p1), i
+
+
 def test_batched_mixed_precision_error(mf, gpu):
     from minifem.batched import Batch
     print("[batched] mixed", flush=True)
