@@ -278,7 +278,7 @@ def rel_fro(a, b):
     import numpy as np
     m = np.isfinite(b)
     n = float(np.linalg.norm(b[m]))
-    d = float(np.linalg.norm((a - b)[m]))
+    d = float(np.linalg.norm(a[m] - b[m]))
     return d / n if n > 0 else d
 
 
