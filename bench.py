@@ -54,6 +54,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     # schedule of the tiled plan, as tuned for the 100^3 ela mesh on MI355X (the library's
     # own defaults suit smaller meshes; minifem.tune picks per mesh)
+    ap.add_argument("--tune", action="store_true",
+                    help="pick the tiled schedule with the autotuner on this mesh first")
     ap.add_argument("--tile-lds", type=int, default=155000)
     ap.add_argument("--block", type=int, default=1024)
     ap.add_argument("--max-rows", type=int, default=254)
@@ -119,11 +121,17 @@ def main():
         f"{mesh.nbColors} colors ({time.time() - t0:.1f}s)")
     dm = gpu.DeviceMesh(mesh)
     plan = None
+    tuning = None
     if args.schedule == "tiled":
+        params = dict(tile_lds_bytes=args.tile_lds, block_threads=args.block,
+                      max_tile_rows=args.max_rows)
+        if args.tune:
+            from minifem.tune import tune
+            params, res = tune(mesh, dm, precision=prec_mode, save=False, log=log)
+            tuning = [dict(c, us=round(us, 2), tiles=st["tiles"]) for c, us, st in res]
         t1 = time.time()
-        plan = gpu.Plan(mesh, tile_lds_bytes=args.tile_lds, block_threads=args.block,
-                        max_tile_rows=args.max_rows)
-        log(f"plan: {plan.stats()} ({time.time() - t1:.1f}s)")
+        plan = gpu.Plan(mesh, **params)
+        log(f"plan: {params} {plan.stats()} ({time.time() - t1:.1f}s)")
     stream = gpu.Stream()
 
     out_bytes = 8 * mesh.nnz * (9 if op == minifem.OP_ELA else 1)
@@ -264,11 +272,13 @@ def main():
                                f"{mesh.nbNodes} nodes, {mesh.nbElem} tets, {mesh.nnz} CSR blocks; "
                                "assembly + block-Jacobi prec per step",
                    "schedule": args.schedule, "precision": args.precision,
+                   "tile_params": params if plan is not None else None,
                    "wall_s": round(wall, 4)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_baseline_coloring_omp": cpu_omp,
         "accuracy": accuracy,
+        "tuning": tuning,
     }
     print(json.dumps(out), flush=True)
 

@@ -13,9 +13,12 @@ from . import gpu
 
 TUNED_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 
+# small budgets give small meshes enough tiles to fill 256 CUs; large ones amortize the
+# per-tile chain on large meshes
 DEFAULT_CANDIDATES = [
     dict(tile_lds_bytes=lds, block_threads=bt, max_tile_rows=254)
-    for lds, bt in ((48 * 1024, 512), (64 * 1024, 512), (78 * 1024, 512), (100 * 1024, 1024),
+    for lds, bt in ((16 * 1024, 512), (24 * 1024, 512), (32 * 1024, 512), (48 * 1024, 512),
+                    (64 * 1024, 512), (78 * 1024, 512), (100 * 1024, 1024),
                     (120 * 1024, 1024), (150 * 1024, 1024), (155000, 1024))
 ]
 
