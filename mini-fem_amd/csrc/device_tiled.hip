@@ -12,6 +12,7 @@
 
 #include "device_common.h"
 #include "minifem.h"
+#include "abi_guard.h"
 #include "plan.h"
 
 struct mf_plan {
@@ -893,49 +894,51 @@ extern "C" int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nb
                               const int *checkBounds, const double *coord,
                               const mf_plan_params *params)
 {
-    if (!plan) return MF_ERR_ARG;
-    *plan = nullptr;
-    if (!valid_args(operatorID, nbElem, nbNodes, nnz, elemToNode, row, col)) return MF_ERR_ARG;
-    Resolved r;
-    int st = resolve_params(operatorID, params, &r);
-    if (st != MF_OK) return st;
-    mf::PlanHost h;
-    int lds64 = 0, lds32 = 0;
-    st = build_checked(h, operatorID, nbElem, nbNodes, nnz, elemToNode, row, col, checkBounds,
-                       coord, r, &lds64, &lds32);
-    if (st != MF_OK) return st;
+    return mf_guarded([&]() -> int {
+        if (!plan) return MF_ERR_ARG;
+        *plan = nullptr;
+        if (!valid_args(operatorID, nbElem, nbNodes, nnz, elemToNode, row, col)) return MF_ERR_ARG;
+        Resolved r;
+        int st = resolve_params(operatorID, params, &r);
+        if (st != MF_OK) return st;
+        mf::PlanHost h;
+        int lds64 = 0, lds32 = 0;
+        st = build_checked(h, operatorID, nbElem, nbNodes, nnz, elemToNode, row, col, checkBounds,
+                           coord, r, &lds64, &lds32);
+        if (st != MF_OK) return st;
 
-    mf_plan *p = new (std::nothrow) mf_plan;
-    if (!p) return MF_ERR_NOMEM;
-    p->op = operatorID;
-    p->nE = nbElem;
-    p->N = nbNodes;
-    p->nnz = nnz;
-    p->dim = operatorID == MF_OP_ELA ? 9 : 1;
-    p->ntiles = (int)h.tiles.size();
-    p->block = r.block;
-    p->lds_fp64 = lds64;
-    p->lds_mixed = lds32;
-    p->n_isolated = (int)h.isolated.size();
-    p->max_node = h.max_node;
-    p->max_elem = h.max_elem;
-    int rc = upload(&p->tiles, h.tiles);
-    if (rc == MF_OK) rc = upload(&p->tile_elems, h.tile_elems);
-    if (rc == MF_OK) rc = upload_as(&p->tile_verts, h.tile_verts);
-    if (rc == MF_OK) rc = upload(&p->tile_nodes, h.tile_nodes);
-    if (rc == MF_OK) rc = upload_as(&p->tile_rowinfo, h.tile_rowinfo);
-    if (rc == MF_OK) rc = upload(&p->inc, h.inc);
-    if (rc == MF_OK) rc = upload_as(&p->task_rec, h.task_rec);
-    if (rc == MF_OK) rc = upload(&p->isolated, h.isolated);
-    if (rc == MF_OK && checkBounds && operatorID == MF_OP_ELA)
-        rc = upload(&p->cb, std::vector<int32_t>(checkBounds, checkBounds + (size_t)nbNodes * 3));
-    if (rc != MF_OK) {
-        free_plan(p);
-        return rc;
-    }
-    fill_stats(h, lds64, r.block, p->stats);
-    *plan = p;
-    return MF_OK;
+        mf_plan *p = new (std::nothrow) mf_plan;
+        if (!p) return MF_ERR_NOMEM;
+        p->op = operatorID;
+        p->nE = nbElem;
+        p->N = nbNodes;
+        p->nnz = nnz;
+        p->dim = operatorID == MF_OP_ELA ? 9 : 1;
+        p->ntiles = (int)h.tiles.size();
+        p->block = r.block;
+        p->lds_fp64 = lds64;
+        p->lds_mixed = lds32;
+        p->n_isolated = (int)h.isolated.size();
+        p->max_node = h.max_node;
+        p->max_elem = h.max_elem;
+        int rc = upload(&p->tiles, h.tiles);
+        if (rc == MF_OK) rc = upload(&p->tile_elems, h.tile_elems);
+        if (rc == MF_OK) rc = upload_as(&p->tile_verts, h.tile_verts);
+        if (rc == MF_OK) rc = upload(&p->tile_nodes, h.tile_nodes);
+        if (rc == MF_OK) rc = upload_as(&p->tile_rowinfo, h.tile_rowinfo);
+        if (rc == MF_OK) rc = upload(&p->inc, h.inc);
+        if (rc == MF_OK) rc = upload_as(&p->task_rec, h.task_rec);
+        if (rc == MF_OK) rc = upload(&p->isolated, h.isolated);
+        if (rc == MF_OK && checkBounds && operatorID == MF_OP_ELA)
+            rc = upload(&p->cb, std::vector<int32_t>(checkBounds, checkBounds + (size_t)nbNodes * 3));
+        if (rc != MF_OK) {
+            free_plan(p);
+            return rc;
+        }
+        fill_stats(h, lds64, r.block, p->stats);
+        *plan = p;
+        return MF_OK;
+    });
 }
 
 extern "C" int mf_plan_host_stats(int operatorID, int nbElem, int nbNodes, int nnz,
@@ -943,18 +946,20 @@ extern "C" int mf_plan_host_stats(int operatorID, int nbElem, int nbNodes, int n
                                   const double *coord, const mf_plan_params *params,
                                   int64_t out[8])
 {
-    if (!out || !valid_args(operatorID, nbElem, nbNodes, nnz, elemToNode, row, col))
-        return MF_ERR_ARG;
-    Resolved r;
-    int st = resolve_params(operatorID, params, &r);
-    if (st != MF_OK) return st;
-    mf::PlanHost h;
-    int lds64 = 0, lds32 = 0;
-    st = build_checked(h, operatorID, nbElem, nbNodes, nnz, elemToNode, row, col, nullptr,
-                       coord, r, &lds64, &lds32);
-    if (st != MF_OK) return st;
-    fill_stats(h, lds64, r.block, out);
-    return MF_OK;
+    return mf_guarded([&]() -> int {
+        if (!out || !valid_args(operatorID, nbElem, nbNodes, nnz, elemToNode, row, col))
+            return MF_ERR_ARG;
+        Resolved r;
+        int st = resolve_params(operatorID, params, &r);
+        if (st != MF_OK) return st;
+        mf::PlanHost h;
+        int lds64 = 0, lds32 = 0;
+        st = build_checked(h, operatorID, nbElem, nbNodes, nnz, elemToNode, row, col, nullptr,
+                           coord, r, &lds64, &lds32);
+        if (st != MF_OK) return st;
+        fill_stats(h, lds64, r.block, out);
+        return MF_OK;
+    });
 }
 
 extern "C" int mf_plan_destroy(mf_plan *plan)

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "minifem.h"
+#include "abi_guard.h"
 
 namespace {
 
@@ -66,22 +67,24 @@ extern "C" int mf_store_input_data(const char *path, const int header[6],
                                    const int *neighborsList, const int *intfIndex,
                                    const int *intfNodes, const int *boundNodesCode)
 {
-    if (!path || !header) return MF_ERR_ARG;
-    for (int i = 0; i < 6; i++)
-        if (header[i] < 0) return MF_ERR_ARG;
-    File out(std::fopen(path, "wb"));
-    if (!out.f) return MF_ERR_IO;
-    const int64_t nE = header[0], nN = header[1], nI = header[3], nIN = header[4];
-    const int64_t nl = (nI > 1 ? nI : 1) * 3;
-    // A single rank has no interface: the reference still stores max(nbIntf,1)*3
-    // neighbour ints and nbIntf+1 index ints (src/IO.cc:122-123).
-    std::vector<int> zeros((size_t)(nl > nI + 1 ? nl : nI + 1), 0);
-    bool ok = wr(out.f, header, 6) && wr(out.f, coord, nN * 3) &&
-              wr(out.f, elemToNode, nE * 4) &&
-              wr(out.f, neighborsList ? neighborsList : zeros.data(), nl) &&
-              wr(out.f, intfIndex ? intfIndex : zeros.data(), nI + 1) &&
-              wr(out.f, intfNodes, nIN) && wr(out.f, boundNodesCode, nN);
-    return ok ? MF_OK : MF_ERR_IO;
+    return mf_guarded([&]() -> int {
+        if (!path || !header) return MF_ERR_ARG;
+        for (int i = 0; i < 6; i++)
+            if (header[i] < 0) return MF_ERR_ARG;
+        File out(std::fopen(path, "wb"));
+        if (!out.f) return MF_ERR_IO;
+        const int64_t nE = header[0], nN = header[1], nI = header[3], nIN = header[4];
+        const int64_t nl = (nI > 1 ? nI : 1) * 3;
+        // A single rank has no interface: the reference still stores max(nbIntf,1)*3
+        // neighbour ints and nbIntf+1 index ints (src/IO.cc:122-123).
+        std::vector<int> zeros((size_t)(nl > nI + 1 ? nl : nI + 1), 0);
+        bool ok = wr(out.f, header, 6) && wr(out.f, coord, nN * 3) &&
+                  wr(out.f, elemToNode, nE * 4) &&
+                  wr(out.f, neighborsList ? neighborsList : zeros.data(), nl) &&
+                  wr(out.f, intfIndex ? intfIndex : zeros.data(), nI + 1) &&
+                  wr(out.f, intfNodes, nIN) && wr(out.f, boundNodesCode, nN);
+        return ok ? MF_OK : MF_ERR_IO;
+    });
 }
 
 extern "C" int mf_read_ref_assembly(const char *path, double *matrixNorm, double *precNorm)
@@ -167,67 +170,69 @@ extern "C" int mf_cube_sizes(int n, int *nbElem, int *nbNodes, int64_t *nnz)
 extern "C" int mf_generate_cube(int n, double jitter, uint64_t geomSeed, uint64_t shuffleSeed,
                                 double *coord, int *elemToNode, int *boundNodesCode)
 {
-    int nE, nN;
-    if (mf_cube_sizes(n, &nE, &nN, nullptr) != MF_OK || !coord || !elemToNode ||
-        !boundNodesCode || jitter < 0 || jitter >= 0.5)
-        return MF_ERR_ARG;
-    const double h = 1.0 / (n - 1);
-    Rng geo(geomSeed);
-    auto id = [n](int i, int j, int k) { return i + n * (j + n * k); };
-    for (int k = 0; k < n; k++)
-        for (int j = 0; j < n; j++)
-            for (int i = 0; i < n; i++) {
-                const int v = id(i, j, k);
-                double *c = coord + (int64_t)v * 3;
-                c[0] = i * h;
-                c[1] = j * h;
-                c[2] = k * h;
-                const bool interior = i > 0 && j > 0 && k > 0 && i < n - 1 && j < n - 1 &&
-                                      k < n - 1;
-                if (interior)
-                    for (int d = 0; d < 3; d++) c[d] += (2.0 * geo.uniform() - 1.0) * jitter * h;
-                int code = 0;
-                if (i == n - 1 || j == n - 1 || k == n - 1) code = 10;
-                else if (i == 0) code = 52;
-                else if (j == 0) code = 53;
-                else if (k == 0) code = 54;
-                boundNodesCode[v] = code;
-            }
-    int64_t e = 0;
-    for (int k = 0; k < n - 1; k++)
-        for (int j = 0; j < n - 1; j++)
-            for (int i = 0; i < n - 1; i++)
-                for (int t = 0; t < 6; t++) {
-                    int p[3] = {i, j, k};
-                    int *out = elemToNode + e * 4;
-                    out[0] = id(p[0], p[1], p[2]) + 1;
-                    p[kAxisOrder[t][0]]++;
-                    out[1] = id(p[0], p[1], p[2]) + 1;
-                    p[kAxisOrder[t][1]]++;
-                    out[2] = id(p[0], p[1], p[2]) + 1;
-                    out[3] = id(i + 1, j + 1, k + 1) + 1;
-                    e++;
+    return mf_guarded([&]() -> int {
+        int nE, nN;
+        if (mf_cube_sizes(n, &nE, &nN, nullptr) != MF_OK || !coord || !elemToNode ||
+            !boundNodesCode || jitter < 0 || jitter >= 0.5)
+            return MF_ERR_ARG;
+        const double h = 1.0 / (n - 1);
+        Rng geo(geomSeed);
+        auto id = [n](int i, int j, int k) { return i + n * (j + n * k); };
+        for (int k = 0; k < n; k++)
+            for (int j = 0; j < n; j++)
+                for (int i = 0; i < n; i++) {
+                    const int v = id(i, j, k);
+                    double *c = coord + (int64_t)v * 3;
+                    c[0] = i * h;
+                    c[1] = j * h;
+                    c[2] = k * h;
+                    const bool interior = i > 0 && j > 0 && k > 0 && i < n - 1 && j < n - 1 &&
+                                          k < n - 1;
+                    if (interior)
+                        for (int d = 0; d < 3; d++) c[d] += (2.0 * geo.uniform() - 1.0) * jitter * h;
+                    int code = 0;
+                    if (i == n - 1 || j == n - 1 || k == n - 1) code = 10;
+                    else if (i == 0) code = 52;
+                    else if (j == 0) code = 53;
+                    else if (k == 0) code = 54;
+                    boundNodesCode[v] = code;
                 }
-    if (shuffleSeed != 0) {
-        // Fisher-Yates: newId[old] is a seeded random permutation of the node ids.
-        std::vector<int> newId((size_t)nN);
-        for (int v = 0; v < nN; v++) newId[v] = v;
-        Rng sh(shuffleSeed);
-        for (int v = nN - 1; v > 0; v--) {
-            const int u = (int)sh.below((uint64_t)v + 1);
-            std::swap(newId[v], newId[u]);
+        int64_t e = 0;
+        for (int k = 0; k < n - 1; k++)
+            for (int j = 0; j < n - 1; j++)
+                for (int i = 0; i < n - 1; i++)
+                    for (int t = 0; t < 6; t++) {
+                        int p[3] = {i, j, k};
+                        int *out = elemToNode + e * 4;
+                        out[0] = id(p[0], p[1], p[2]) + 1;
+                        p[kAxisOrder[t][0]]++;
+                        out[1] = id(p[0], p[1], p[2]) + 1;
+                        p[kAxisOrder[t][1]]++;
+                        out[2] = id(p[0], p[1], p[2]) + 1;
+                        out[3] = id(i + 1, j + 1, k + 1) + 1;
+                        e++;
+                    }
+        if (shuffleSeed != 0) {
+            // Fisher-Yates: newId[old] is a seeded random permutation of the node ids.
+            std::vector<int> newId((size_t)nN);
+            for (int v = 0; v < nN; v++) newId[v] = v;
+            Rng sh(shuffleSeed);
+            for (int v = nN - 1; v > 0; v--) {
+                const int u = (int)sh.below((uint64_t)v + 1);
+                std::swap(newId[v], newId[u]);
+            }
+            std::vector<double> c2((size_t)nN * 3);
+            std::vector<int> b2((size_t)nN);
+            for (int v = 0; v < nN; v++) {
+                std::memcpy(&c2[(size_t)newId[v] * 3], coord + (int64_t)v * 3, 3 * sizeof(double));
+                b2[newId[v]] = boundNodesCode[v];
+            }
+            std::memcpy(coord, c2.data(), c2.size() * sizeof(double));
+            std::memcpy(boundNodesCode, b2.data(), b2.size() * sizeof(int));
+            for (int64_t q = 0; q < (int64_t)nE * 4; q++) elemToNode[q] = newId[elemToNode[q] - 1] + 1;
         }
-        std::vector<double> c2((size_t)nN * 3);
-        std::vector<int> b2((size_t)nN);
-        for (int v = 0; v < nN; v++) {
-            std::memcpy(&c2[(size_t)newId[v] * 3], coord + (int64_t)v * 3, 3 * sizeof(double));
-            b2[newId[v]] = boundNodesCode[v];
-        }
-        std::memcpy(coord, c2.data(), c2.size() * sizeof(double));
-        std::memcpy(boundNodesCode, b2.data(), b2.size() * sizeof(int));
-        for (int64_t q = 0; q < (int64_t)nE * 4; q++) elemToNode[q] = newId[elemToNode[q] - 1] + 1;
-    }
-    return MF_OK;
+        return MF_OK;
+    });
 }
 
 extern "C" int mf_generate_coef(int operatorID, int nbElem, uint64_t seed, double *coef)

@@ -208,10 +208,15 @@ struct Partitioner {
                 spawned = true;
             } catch (...) {
             }
-            if (!spawned) rcb(lo, mid, s, left, depth + 1);
             std::vector<Leaf> right;
-            rcb(mid, hi, s, right, depth + 1);
+            try {
+                if (!spawned) rcb(lo, mid, s, left, depth + 1);
+                rcb(mid, hi, s, right, depth + 1);
+            } catch (...) {  // join the other half before anything leaves this frame
+                failed = 1;
+            }
             if (spawned) t.join();
+            if (failed) return;
             out.insert(out.end(), left.begin(), left.end());
             out.insert(out.end(), right.begin(), right.end());
             return;
@@ -499,6 +504,7 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
     const int nc = std::max(1, std::min(threads, T / 64));
     std::vector<TileChunk> chunks((size_t)nc);
     std::vector<std::thread> pool;
+    pool.reserve((size_t)nc);
     for (int c = 1; c < nc; c++) {
         try {
             pool.emplace_back([&, c] { em.emit_range((int)((int64_t)T * c / nc),
@@ -507,7 +513,7 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
             em.emit_range((int)((int64_t)T * c / nc), (int)((int64_t)T * (c + 1) / nc), chunks[c]);
         }
     }
-    em.emit_range(0, (int)((int64_t)T / nc), chunks[0]);
+    em.emit_range(0, (int)((int64_t)T / nc), chunks[0]);  // catches its own exceptions
     for (auto &t : pool) t.join();
     size_t nTiles = 0, nElems = 0, nNodes = 0, nRinfo = 0, nInc = 0, nTask = 0;
     for (const TileChunk &c : chunks) {
@@ -734,6 +740,7 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
         }
     };
     std::vector<std::thread> pool;
+    pool.reserve((size_t)nc);
     for (int c = 1; c < nc; c++) {
         try {
             pool.emplace_back(run, c);
