@@ -255,9 +255,9 @@ typedef struct mf_plan mf_plan;
  * when a and c are rows of one tile the plan computes the pair once and writes both;
  * a tile computes every block of its rows itself, so each tile writes whole CSR rows. */
 typedef struct mf_plan_params {
-    int tile_lds_bytes;   /* LDS budget of one tile: 4096..163840 (default 65536) */
-    int block_threads;    /* threads per workgroup: 512 or 1024 (default 512) */
-    int max_tile_rows;    /* cap on rows per tile, <= 256 (default 64 ela, 128 lap) */
+    int tile_lds_bytes;   /* LDS budget of one tile: 4096..163840 (default 155000) */
+    int block_threads;    /* threads per workgroup: 512 or 1024 (default 1024) */
+    int max_tile_rows;    /* cap on rows per tile, <= 256 (default 254) */
 } mf_plan_params;
 
 /* Host arrays: elemToNode (color-permuted, 1-based, 4nE), row[N+1], col[nnz],
