@@ -21,7 +21,7 @@ struct mf_plan {
     int max_node = 0, max_elem = 0;  // the largest tile: sizes the fixed front of its LDS
     int lds_fp64 = 0, lds_mixed = 0;
     int n_isolated = 0;
-    int grid[3] = {0, 0, 0};  // persistent grid per precision mode, set on its first launch
+    int grid[3] = {0, 0, 0};  // persistent grid per precision mode, set at creation
     int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     mf::TileHdr *tiles = nullptr;
     int32_t *tile_elems = nullptr;
@@ -766,6 +766,31 @@ void free_plan(mf_plan *p)
     delete p;
 }
 
+// At plan creation, for each precision mode: allow the whole LDS for this kernel (the
+// attribute is per kernel, shared by every plan), then the persistent grid = as many
+// workgroups per XCD as fit at once, at most one per slot of its tile range. Launches
+// then only read the plan: they make no runtime queries (also under graph capture), and
+// several host threads may launch one plan at once.
+template <int OP, int PREC, int BT>
+int prepare_tiled(mf_plan *p)
+{
+    if (p->ntiles == 0) return MF_OK;
+    const int lds = PREC == MF_PREC_FP64 ? p->lds_fp64 : p->lds_mixed;
+    auto kern = k_tiled<OP, PREC, BT>;
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            mf::kMaxLdsBytes) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(kern), BT, lds) != hipSuccess)
+        return MF_ERR_HIP;
+    const int per = (p->ntiles + kXcd - 1) / kXcd;
+    p->grid[PREC] = std::max(1, std::min(per, std::max(1, per_cu) * std::max(1, cus / kXcd)));
+    return MF_OK;
+}
+
 template <int OP, int PREC, int BT>
 int launch_tiled(mf_plan *p, const double *coord, const double *coef, double *values,
                  double *prec, int *singular, hipStream_t s)
@@ -773,24 +798,8 @@ int launch_tiled(mf_plan *p, const double *coord, const double *coef, double *va
     const int lds = PREC == MF_PREC_FP64 ? p->lds_fp64 : p->lds_mixed;
     auto kern = k_tiled<OP, PREC, BT>;
     if (p->ntiles > 0) {
-        int &G = p->grid[PREC];
-        if (G == 0) {
-            // first launch: allow the whole LDS for this kernel (the attribute is per
-            // kernel, shared by every plan), then the persistent grid = as many workgroups
-            // per XCD as fit at once, at most one per slot of its tile range. Later
-            // launches (also under graph capture) make no runtime queries.
-            int dev = 0, cus = 0, per_cu = 0;
-            if (hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    mf::kMaxLdsBytes) != hipSuccess ||
-                hipGetDevice(&dev) != hipSuccess ||
-                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-                hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per_cu, reinterpret_cast<const void *>(kern), BT, lds) != hipSuccess)
-                return MF_ERR_HIP;
-            const int per = (p->ntiles + kXcd - 1) / kXcd;
-            G = std::max(1, std::min(per, std::max(1, per_cu) * std::max(1, cus / kXcd)));
-        }
+        const int G = p->grid[PREC];
+        if (G < 1) return MF_ERR_STATE;  // not prepared at creation
         hipLaunchKernelGGL(kern, dim3(G * kXcd), dim3(BT), lds, s, p->tiles, p->ntiles, G,
                            p->max_node, p->max_elem,
                            p->tile_elems, p->tile_verts, p->tile_nodes, p->tile_rowinfo,
@@ -804,6 +813,25 @@ int launch_tiled(mf_plan *p, const double *coord, const double *coef, double *va
         MF_CHECK_LAUNCH();
     }
     return MF_OK;
+}
+
+template <int OP, int PREC>
+int prepare_variant(mf_plan *p)
+{
+    switch (p->block) {
+    case 512: return prepare_tiled<OP, PREC, 512>(p);
+    case 1024: return prepare_tiled<OP, PREC, 1024>(p);
+    default: return MF_ERR_ARG;
+    }
+}
+
+template <int OP>
+int prepare_all(mf_plan *p)
+{
+    int st = prepare_variant<OP, MF_PREC_FP64>(p);
+    if (st == MF_OK) st = prepare_variant<OP, MF_PREC_MIXED>(p);
+    if (st == MF_OK) st = prepare_variant<OP, MF_PREC_MIXED_F32COEF>(p);
+    return st;
 }
 
 template <int OP, int PREC>
@@ -932,6 +960,8 @@ extern "C" int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nb
         if (rc == MF_OK) rc = upload(&p->isolated, h.isolated);
         if (rc == MF_OK && checkBounds && operatorID == MF_OP_ELA)
             rc = upload(&p->cb, std::vector<int32_t>(checkBounds, checkBounds + (size_t)nbNodes * 3));
+        if (rc == MF_OK)
+            rc = operatorID == MF_OP_ELA ? prepare_all<MF_OP_ELA>(p) : prepare_all<MF_OP_LAP>(p);
         if (rc != MF_OK) {
             free_plan(p);
             return rc;
