@@ -289,7 +289,9 @@ int mf_plan_stats(const mf_plan *plan, int64_t out[8]);
  * (prec_init + prec_inversion, src/FEM.cc:199-232) in one launch. Device: coord[3N],
  * coef (NULL = reference constants; points to float with MF_PREC_MIXED_F32COEF),
  * values[nnz*d], prec[N*d], singular (may be NULL; incremented per singular ela
- * block). */
+ * block). The plan is only read: one plan may be launched from several host threads and
+ * on several streams at once (each launch with its own outputs), and under graph
+ * capture (no runtime queries). */
 int mf_plan_assemble(mf_plan *plan, const double *coord, const double *coef,
                      double *values, double *prec, int precision, int *singular,
                      void *stream);
