@@ -144,8 +144,9 @@ class Batch:
         self.stream.sync()
 
     def capture(self):
-        """Builds the batch graph (after one eager run, which also fixes each plan's
-        launch grid, so the captured launches make no runtime queries)."""
+        """Builds the batch graph after one eager run (every output then holds a result
+        before the first replay). The plans fixed their launch grids at creation, so the
+        captured launches make no runtime queries."""
         self.run_eager()
         self.graph = gpu.Graph(self.stream, self.enqueue)
         return self
