@@ -259,3 +259,36 @@ def test_tiled_bit_exact_bench_workload(mf, orc, gpu):
     dm.values.fill(np.nan)
     plan.assemble(dm.coord, dm.coef, dm.values, dm.prec, precision=mf.PREC_MIXED)
     assert np.array_equal(dm.values.numpy(), om.assemble(mixed=True))
+
+
+def test_plan_launched_from_two_host_threads(mf, orc, gpu):
+    """One plan launched from two host threads at once, each on its own stream with its
+    own outputs: every result equals a single launch bit for bit."""
+    import threading
+    print("[tiled] two host threads", flush=True)
+    mesh, _ = build(mf, orc, 20, 1, jitter=0.25, seed=3, shuffle=7, coef_seed=5, oracle=False)
+    plan, dm = run_plan(gpu, mesh)
+    want_v, want_p = dm.values.numpy(), dm.prec.numpy()
+    outs = [(gpu.DeviceArray(mesh.nnz * 9, np.float64), gpu.DeviceArray(mesh.nbNodes * 9, np.float64))
+            for _ in range(2)]
+    errors = []
+
+    def work(i):
+        try:
+            gpu.init(0)
+            s = gpu.Stream()
+            v, p = outs[i]
+            for _ in range(5):
+                plan.assemble(dm.coord, dm.coef, v, p, stream=s)
+            s.sync()
+        except Exception as exc:  # reported by the main thread
+            errors.append(exc)
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for v, p in outs:
+        assert np.array_equal(v.numpy(), want_v) and np.array_equal(p.numpy(), want_p)
