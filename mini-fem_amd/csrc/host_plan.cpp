@@ -660,9 +660,9 @@ struct TileValidator {
                 const int c = col[row[a] + oa] - 1;
                 if (c == a || !claim(hit + row[a] + oa)) return MF_ERR_STATE;
                 const int ec = localOf.get(c);
-                const int cnt = ec < 0 ? 0 : __builtin_popcountll(holds[ec]);
-                if (ec < 0 || cl != ec || cnt < 1) return MF_ERR_STATE;
-                if (cnt <= kListLen) {
+                const int nHold = ec < 0 ? 0 : __builtin_popcountll(holds[ec]);
+                if (ec < 0 || cl != ec || nHold < 1) return MF_ERR_STATE;
+                if (nHold <= kListLen) {
                     // the exact ascending list of (element, slot of a, slot of c), padded
                     // with the zero slot
                     int q = 0;
