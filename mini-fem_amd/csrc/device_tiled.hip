@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -936,8 +937,14 @@ extern "C" int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nb
                            coord, r, &lds64, &lds32);
         if (st != MF_OK) return st;
 
-        mf_plan *p = new (std::nothrow) mf_plan;
-        if (!p) return MF_ERR_NOMEM;
+        // every host container is built before the first device allocation, and the plan
+        // is held by a guard until it is complete: an exception thrown here (bad_alloc)
+        // releases whatever device memory the plan already holds
+        std::vector<int32_t> cbHost;
+        if (checkBounds && operatorID == MF_OP_ELA)
+            cbHost.assign(checkBounds, checkBounds + (size_t)nbNodes * 3);
+        std::unique_ptr<mf_plan, void (*)(mf_plan *)> guard(new mf_plan, free_plan);
+        mf_plan *p = guard.get();
         p->op = operatorID;
         p->nE = nbElem;
         p->N = nbNodes;
@@ -958,16 +965,12 @@ extern "C" int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nb
         if (rc == MF_OK) rc = upload(&p->inc, h.inc);
         if (rc == MF_OK) rc = upload_as(&p->task_rec, h.task_rec);
         if (rc == MF_OK) rc = upload(&p->isolated, h.isolated);
-        if (rc == MF_OK && checkBounds && operatorID == MF_OP_ELA)
-            rc = upload(&p->cb, std::vector<int32_t>(checkBounds, checkBounds + (size_t)nbNodes * 3));
+        if (rc == MF_OK && !cbHost.empty()) rc = upload(&p->cb, cbHost);
         if (rc == MF_OK)
             rc = operatorID == MF_OP_ELA ? prepare_all<MF_OP_ELA>(p) : prepare_all<MF_OP_LAP>(p);
-        if (rc != MF_OK) {
-            free_plan(p);
-            return rc;
-        }
+        if (rc != MF_OK) return rc;  // the guard frees the plan
         fill_stats(h, lds64, r.block, p->stats);
-        *plan = p;
+        *plan = guard.release();
         return MF_OK;
     });
 }

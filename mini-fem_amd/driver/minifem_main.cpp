@@ -5,7 +5,10 @@
 //   minifem <mesh> <lap|ela> <iters> [options]
 //     --data DIR          data root (the reference bakes DATA_PATH in at build time);
 //                         default $MINIFEM_DATA or ./data
-//     --schedule S        colored (one launch per color) | tiled (fused plan); default tiled
+//     --schedule S        tiled (default): mf_plan_create once, then one fused
+//                         mf_plan_assemble launch per iteration (assembly + prec_init +
+//                         prec_inversion); colored: the drop-in calls mf_assembly_ex,
+//                         mf_prec_init, mf_prec_inversion (one launch per color)
 //     --precision P       fp64 | mixed; default fp64
 //     --coef FILE         per-element coefficients (raw float64, file element order)
 //     --store-check       write checkings/<op>_1_0 from this run instead of reading it
@@ -13,7 +16,9 @@
 //     writes DIR/<mesh>/inputs/<op>_1_0 for an n^3 jittered Kuhn cube
 //
 // Input: DIR/<mesh>/inputs/<op>_1_0 (src/IO.cc:66-68). Check:
-// DIR/<mesh>/checkings/<op>_1_0 (src/IO.cc:29-31), results in numerical_results_0.
+// DIR/<mesh>/checkings/<op>_1_0 (src/IO.cc:29-31), results in numerical_results_0. The
+// norms are taken on the host in the reference's serial order (compute_double_norm,
+// src/FEM.cc:48-56) after the last iteration's values and prec are copied back.
 #include <hip/hip_runtime_api.h>
 
 #include <chrono>
@@ -125,6 +130,11 @@ int main(int argc, char **argv)
         else if (!std::strcmp(argv[i], "--data")) i++;
         else die(std::string("unknown option ") + argv[i]);
     }
+    if (schedule != "tiled" && schedule != "colored")
+        die("unknown schedule \"" + schedule + "\" (tiled or colored)");
+    if (precName != "fp64" && precName != "mixed")
+        die("unknown precision \"" + precName + "\" (fp64 or mixed)");
+    const bool tiled = schedule == "tiled";
     const int opID = opName == "lap" ? MF_OP_LAP : MF_OP_ELA, dim = opID ? 9 : 1;
     const int prec = precName == "mixed" ? MF_PREC_MIXED : MF_PREC_FP64;
 
@@ -188,6 +198,19 @@ int main(int argc, char **argv)
     if (hipMalloc(reinterpret_cast<void **>(&dValues), (size_t)nnz * dim * sizeof(double)) != hipSuccess ||
         hipMalloc(reinterpret_cast<void **>(&dPrec), (size_t)nN * dim * sizeof(double)) != hipSuccess)
         die("hipMalloc failed");
+    // tiled: the gather schedule is built once per topology (host arrays: the permuted
+    // elemToNode, the CSR pattern, the mask; coordinates group nodes into tiles)
+    mf_plan *plan = nullptr;
+    if (tiled) {
+        t = clk::now();
+        check(mf_plan_create(&plan, opID, nE, nN, nnz, e2n.data(), row.data(), col.data(),
+                             cb.data(), coord.data(), nullptr),
+              "mf_plan_create");
+        int64_t st[8];
+        check(mf_plan_stats(plan, st), "mf_plan_stats");
+        std::printf("Building the tiled plan...           done  (%.1e seconds, %lld tiles)\n",
+                    secs(t), (long long)st[0]);
+    }
     hipStream_t stream;
     hipck(hipStreamCreate(&stream), "hipStreamCreate");
     hipEvent_t ev[5];
@@ -198,16 +221,25 @@ int main(int argc, char **argv)
     int timed = 0;
     for (int it = 0; it < nbIter; it++) {
         hipck(hipEventRecord(ev[0], stream), "hipEventRecord");
-        check(mf_assembly_ex(dCoord, dCoef, dValues, dRow, dCol, dE2n, nullptr, nE, nnz, dim, opID,
-                             colorToElem.data(), nColors, prec, stream),
-              "assembly");
-        hipck(hipEventRecord(ev[1], stream), "hipEventRecord");
-        check(mf_prec_init(dPrec, dValues, dRow, dCol, nN, dim, stream), "prec_init");
-        hipck(hipEventRecord(ev[2], stream), "hipEventRecord");
-        // halo exchange: nothing to exchange at one rank (src/halo.cc:44)
-        hipck(hipEventRecord(ev[3], stream), "hipEventRecord");
-        check(mf_prec_inversion(dPrec, dRow, dCol, dCb, nN, opID, nullptr, stream), "prec_inversion");
-        hipck(hipEventRecord(ev[4], stream), "hipEventRecord");
+        if (tiled) {
+            // assembly + prec_init + prec_inversion (src/FEM.cc:183-232) in one launch: the
+            // whole step is reported as the assembly phase
+            check(mf_plan_assemble(plan, dCoord, dCoef, dValues, dPrec, prec, nullptr, stream),
+                  "mf_plan_assemble");
+            for (int p = 1; p <= 4; p++) hipck(hipEventRecord(ev[p], stream), "hipEventRecord");
+        } else {
+            check(mf_assembly_ex(dCoord, dCoef, dValues, dRow, dCol, dE2n, nullptr, nE, nnz, dim,
+                                 opID, colorToElem.data(), nColors, prec, stream),
+                  "assembly");
+            hipck(hipEventRecord(ev[1], stream), "hipEventRecord");
+            check(mf_prec_init(dPrec, dValues, dRow, dCol, nN, dim, stream), "prec_init");
+            hipck(hipEventRecord(ev[2], stream), "hipEventRecord");
+            // halo exchange: nothing to exchange at one rank (src/halo.cc:44)
+            hipck(hipEventRecord(ev[3], stream), "hipEventRecord");
+            check(mf_prec_inversion(dPrec, dRow, dCol, dCb, nN, opID, nullptr, stream),
+                  "prec_inversion");
+            hipck(hipEventRecord(ev[4], stream), "hipEventRecord");
+        }
         hipck(hipEventSynchronize(ev[4]), "hipEventSynchronize");
         if (nbIter == 1 || it > 0) {
             for (int p = 0; p < 4; p++) {
@@ -219,6 +251,8 @@ int main(int argc, char **argv)
         }
         std::printf("%d. Matrix assembly / preconditioner  done\n", it);
     }
+    if (tiled) std::printf("\n(tiled: assembly and both preconditioner phases run as one fused "
+                          "launch, timed as the assembly phase)\n");
     std::printf("\nAverage time (microseconds)\n----------------------------------------------\n"
                 "  Matrix assembly               : %.1f\n  Preconditioner initialization : %.1f\n"
                 "  Halo exchange                 : %.1f\n  Preconditioner inversion      : %.1f\n"
@@ -227,10 +261,14 @@ int main(int argc, char **argv)
                 1e3 * phase_ms[3] / timed,
                 1e3 * (phase_ms[0] + phase_ms[1] + phase_ms[2] + phase_ms[3]) / timed);
 
-    // Numerical check (src/FEM.cc:59-98)
-    double mNorm = 0, pNorm = 0;
-    check(mf_norm(dValues, (int64_t)nnz * dim, &mNorm, stream), "norm");
-    check(mf_norm(dPrec, (int64_t)nN * dim, &pNorm, stream), "norm");
+    // Numerical check (src/FEM.cc:59-98): the norms in the reference's serial order
+    std::vector<double> hValues((size_t)nnz * dim), hPrec((size_t)nN * dim);
+    hipck(hipMemcpy(hValues.data(), dValues, hValues.size() * sizeof(double), hipMemcpyDeviceToHost),
+          "hipMemcpy");
+    hipck(hipMemcpy(hPrec.data(), dPrec, hPrec.size() * sizeof(double), hipMemcpyDeviceToHost),
+          "hipMemcpy");
+    const double mNorm = mf_compute_double_norm(hValues.data(), (int64_t)hValues.size());
+    const double pNorm = mf_compute_double_norm(hPrec.data(), (int64_t)hPrec.size());
     const std::string ckdir = root + "/" + mesh + "/checkings", ck = ckdir + "/" + opName + "_1_0";
     if (storeCheck) {
         mkdirs(ckdir);
@@ -252,5 +290,8 @@ int main(int argc, char **argv)
     for (void *p : {(void *)dCoord, (void *)dCoef, (void *)dE2n, (void *)dRow, (void *)dCol,
                     (void *)dCb, (void *)dValues, (void *)dPrec})
         if (p) hipck(hipFree(p), "hipFree");
+    for (auto &e : ev) hipck(hipEventDestroy(e), "hipEventDestroy");
+    hipck(hipStreamDestroy(stream), "hipStreamDestroy");
+    if (plan) check(mf_plan_destroy(plan), "mf_plan_destroy");
     return EXIT_SUCCESS;
 }

@@ -127,6 +127,12 @@ class Batch:
         self._join = [gpu.Event() for _ in range(self.branches)]
         self.graph = None
 
+    def set_precision(self, precision):
+        """Switches the precision mode of every launch (the plans serve every mode); the
+        captured graph, if any, is dropped and captured again on the next launch()."""
+        self.precision = precision
+        self.graph = None
+
     def enqueue(self, stream):
         """Every mesh's iteration, forked over the side streams and joined on `stream`."""
         self._fork.record(stream)

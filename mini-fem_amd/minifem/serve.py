@@ -120,6 +120,10 @@ class Server:
     """Resident meshes + an HBM budget + `slots` concurrent request streams."""
 
     def __init__(self, meshes, hbm_budget, slots=4, precision=PREC_FP64, plan_params=None):
+        if precision not in (PREC_FP64, PREC_MIXED):
+            # MF_PREC_MIXED_F32COEF is chosen per request, for a float32 coefficient
+            # payload staged by a PREC_MIXED server; a float64 field read as float is wrong
+            raise ValueError(f"Server precision must be PREC_FP64 or PREC_MIXED, got {precision}")
         self.meshes = list(meshes)
         self.precision = precision
         self.dms = [gpu.DeviceMesh(m) for m in self.meshes]
@@ -128,7 +132,7 @@ class Server:
         self.arena = Arena(hbm_budget)
         self.slots = [gpu.Stream() for _ in range(max(1, int(slots)))]
         self.events = [gpu.Event() for _ in self.slots]
-        # one warm launch per plan fixes its grid before any timing
+        # one warm launch per plan before any timing (code objects loaded, caches warm)
         for p, dm in zip(self.plans, self.dms):
             p.assemble(dm.coord, dm.coef, dm.values, dm.prec, precision=precision)
         gpu.synchronize()
@@ -181,6 +185,7 @@ class Server:
         busy = [None] * len(self.slots)  # (request, arena block, views)
         results = {}
         done = []
+        memory_waits = set()  # requests that found a slot free but not their HBM
         t0 = time.perf_counter()
         while pending or any(b is not None for b in busy):
             now = time.perf_counter() - t0
@@ -202,6 +207,7 @@ class Server:
                 req = pending[0]
                 blk = self.arena.alloc(self._footprint_aligned(req))
                 if blk is None:
+                    memory_waits.add(req.rid)
                     break  # FIFO: wait for memory to free up
                 pending.popleft()
                 views = self._views(req, blk[0])
@@ -218,7 +224,8 @@ class Server:
                  "p99_ms": float(np.percentile(lat, 99) * 1e3),
                  "mean_ms": float(lat.mean() * 1e3), "makespan_s": float(makespan),
                  "blocks_per_s": float(blocks / makespan),
-                 "requests_per_s": float(len(done) / makespan)}
+                 "requests_per_s": float(len(done) / makespan),
+                 "memory_waits": len(memory_waits)}
         return (stats, results) if keep_results else stats
 
     def run_alone(self, req):

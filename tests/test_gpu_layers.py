@@ -175,3 +175,103 @@ def test_serving_rejects_request_above_budget(mf, gpu):
     srv.stage(reqs)
     with pytest.raises(ValueError):
         srv.run(reqs)
+
+
+def test_batched_full_config(mf, orc, gpu):
+    """The batched workload at its size: 64 jittered cubes of 5k-200k nodes, lap and ela
+    alternating (batch_meshes() defaults). The per-mesh form (one launch per mesh) is the
+    reference; its largest lap and largest ela member are bit-exact against the oracle;
+    the union form equals it bit for bit on every graph replay (FP64), and its mixed
+    form is within 1e-6 relative Frobenius error of FP64 for every mesh."""
+    from minifem.batched import Batch, batch_meshes
+    print("[batched full] 64 meshes, 5k-200k nodes", flush=True)
+    meshes = batch_meshes()
+    assert len(meshes) == 64 and max(m.nbNodes for m in meshes) > 150000
+    per = Batch(meshes, branches=8)
+    per.run_eager()
+    want = per.results()
+    del per
+    for op in (mf.OP_LAP, mf.OP_ELA):
+        i = max((k for k, m in enumerate(meshes) if m.operatorID == op),
+                key=lambda k: meshes[k].nbNodes)
+        m = meshes[i]
+        print(f"[batched full] mesh {i} ({m.nbNodes} nodes, op {op}) vs the oracle", flush=True)
+        om = orc.OracleMesh(m.coord, m.elemToNode_original, m.boundNodesCode, m.nnz, op,
+                            coef=m.coef_original)
+        ov = om.assemble()
+        assert np.array_equal(want[i][0], ov)
+        assert np.array_equal(want[i][1], om.prec(ov)[0])
+        del om, ov
+    u = Batch(meshes, union=True)
+    for prec in (mf.PREC_FP64, mf.PREC_MIXED):
+        u.set_precision(prec)
+        u.capture()
+        for rep in range(2):
+            for dm in u.dms:
+                dm.values.fill(np.nan)
+                dm.prec.fill(np.nan)
+            u.launch()
+            u.stream.sync()
+            got = u.results()
+            for i, ((v0, p0), (v1, p1)) in enumerate(zip(want, got)):
+                if prec == mf.PREC_FP64:
+                    assert np.array_equal(v0, v1) and np.array_equal(p0, p1), (rep, i)
+                else:
+                    ev = np.linalg.norm(v1 - v0) / np.linalg.norm(v0)
+                    ep = np.linalg.norm(p1 - p0) / np.linalg.norm(p0)
+                    assert ev <= 1e-6 and ep <= 1e-6, (rep, i, ev, ep)
+            del got
+            u.run_eager()
+            gpu.synchronize()
+    print(f"[batched full] union: {u.launches()} launch(es) per batch", flush=True)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_served_full_config(mf, orc, gpu, precision):
+    """The served workload at its size: 16 resident meshes of 1k-885k nodes
+    (served_meshes() defaults), a seeded Poisson stream of 32 requests at 20,000/s under
+    an HBM budget below the sum of the footprints, so admission waits for memory. Every
+    request's result in the mix equals the same request served alone, bit for bit; the
+    first coefficient request and the first coordinate request are checked against the
+    oracle (FP64) or within 1e-6 of the FP64 server (mixed)."""
+    from minifem.serve import Server, make_requests, served_meshes
+    print(f"[serve full] 16 meshes, precision {precision}", flush=True)
+    meshes = served_meshes()
+    assert len(meshes) == 16 and max(m.nbNodes for m in meshes) > 800000
+    reqs = make_requests(meshes, count=32, rate_hz=20000.0, seed=5000)
+    fps = [8 * m.dim * (m.nnz + m.nbNodes) + 8 * max(m.coord.size, m.coef.size) for m in meshes]
+    budget = max(int(0.5 * sum(fps)), int(1.05 * max(fps)) + 4096)
+    assert budget < sum(fps)
+    srv = Server(meshes, budget, slots=4, precision=precision)
+    srv.stage(reqs)
+    stats, mixed = srv.run(reqs, keep_results=True)
+    print(f"[serve full] {stats}", flush=True)
+    assert stats["requests"] == len(reqs) and srv.arena.used == 0
+    assert 0 < stats["p50_ms"] <= stats["p99_ms"]
+    for r in reqs:
+        v, p = srv.run_alone(r)
+        assert np.array_equal(v, mixed[r.rid][0]), r.rid
+        assert np.array_equal(p, mixed[r.rid][1]), r.rid
+    sample = [next(r for r in reqs if r.kind == "coef"), next(r for r in reqs if r.kind == "coord")]
+    if precision == mf.PREC_FP64:
+        for r in sample:
+            m = meshes[r.mesh]
+            print(f"[serve full] request {r.rid} ({r.kind}, {m.nbNodes} nodes) vs the oracle",
+                  flush=True)
+            coord = r.payload if r.kind == "coord" else m.coord
+            coef = m.coef_original if r.kind == "coord" else m.unpermute_elements(r.payload)
+            om = orc.OracleMesh(coord, m.elemToNode_original, m.boundNodesCode, m.nnz,
+                                m.operatorID, coef=coef)
+            ov = om.assemble()
+            assert np.array_equal(mixed[r.rid][0], ov)
+            assert np.array_equal(mixed[r.rid][1], om.prec(ov)[0])
+            del om, ov
+    else:
+        del srv
+        ref = Server(meshes, budget, slots=1)
+        ref.stage(sample)
+        for r in sample:
+            v64, p64 = ref.run_alone(r)
+            ev = np.linalg.norm(mixed[r.rid][0] - v64) / np.linalg.norm(v64)
+            ep = np.linalg.norm(mixed[r.rid][1] - p64) / np.linalg.norm(p64)
+            assert ev <= 1e-6 and ep <= 1e-6, (r.rid, ev, ep)
