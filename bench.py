@@ -12,9 +12,13 @@ inverted block-Jacobi prec, with all inputs resident in HBM.
 Prints one JSON line: metric/value (CSR blocks assembled per second), the roofline of
 the fused kernel (algorithmic HBM bytes per launch / its mean launch time, against the
 8 TB/s HBM3E peak), the host-core baselines (the CPU oracle's REF-order iteration and
-its COLORING-order OpenMP iteration on the same mesh, timed here), and the accuracy of
-the GPU results against that oracle iteration (or, with --no-cpu in mixed precision,
-against an FP64 step).
+its COLORING-order OpenMP iteration on the same mesh, both OpenMP variants, timed here),
+the accuracy of the GPU results against that oracle iteration (or, with --no-cpu in
+mixed precision, against an FP64 step), and -- for the default ela 100^3 line -- the
+other two workloads of the metric: "batched" (64 meshes of 5k-200k nodes in one captured
+graph, FP64 and mixed) and "served" (16 resident meshes, a Poisson request stream under an
+HBM budget: p50/p99 latency and throughput of plain FP64 one request at a time against
+continuous batching in FP64 and mixed precision).
 """
 import argparse
 import json
@@ -52,6 +56,11 @@ def main():
     ap.add_argument("--op", default="ela", choices=["ela", "lap"])
     ap.add_argument("--n", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-layers", action="store_true",
+                    help="leave out the batched and served legs of the default line")
+    ap.add_argument("--cpu-iters", type=int, default=10,
+                    help="iterations of the faster (per-color parallel) CPU baseline; the two "
+                         "serial-element-loop baselines run 2 (iteration 0 excluded from all)")
     # schedule of the tiled plan, as tuned for the 100^3 ela mesh on MI355X (the library's
     # own defaults suit smaller meshes; minifem.tune picks per mesh)
     ap.add_argument("--tune", action="store_true",
@@ -59,6 +68,8 @@ def main():
     ap.add_argument("--tile-lds", type=int, default=155000)
     ap.add_argument("--block", type=int, default=1024)
     ap.add_argument("--max-rows", type=int, default=254)
+    ap.add_argument("--id-chunks", type=int, default=1,
+                    help="bisect each of this many contiguous node-id ranges on its own")
     ap.add_argument("--batched", action="store_true",
                     help="instead: 64 meshes (5k-200k nodes, lap/ela) in one captured graph")
     ap.add_argument("--batch-forms", default="per_mesh,union",
@@ -124,7 +135,7 @@ def main():
     tuning = None
     if args.schedule == "tiled":
         params = dict(tile_lds_bytes=args.tile_lds, block_threads=args.block,
-                      max_tile_rows=args.max_rows)
+                      max_tile_rows=args.max_rows, id_chunks=args.id_chunks)
         if args.tune:
             from minifem.tune import tune
             params, res = tune(mesh, dm, precision=prec_mode, save=False, log=log)
@@ -197,14 +208,22 @@ def main():
     kern_s = float(per.mean()) * 1e-3
     achieved = B / kern_s / 1e9
     kname = f"k_tiled<{args.op},{args.precision}>" if plan is not None else "colored step"
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.op}_{args.n}_{args.schedule}_"
-                         f"{args.precision}.json")
+    # HBM traffic per launch is not counted in this run: it is the committed rocprofv3
+    # --pmc summary of the same kernel on the same workload (FETCH_SIZE + WRITE_SIZE per
+    # dispatch, scaled by the copy-kernel calibration the summary states), labelled so
+    traffic = traffic_src = None
+    tname = f"traffic_{args.op}_{args.n}_{args.schedule}_{args.precision}.json"
+    tpath = os.path.join(ROOT, "profiles", tname)
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tj = json.load(f)
+        traffic = tj.get("bytes_per_launch")
+        traffic_src = {"source": f"profiles/{tname} (rocprofv3 --pmc, not measured in this run)",
+                       "measured": tj.get("measured"), "fetch_bytes": tj.get("fetch_bytes"),
+                       "write_bytes": tj.get("write_bytes")}
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_from": traffic_src,
                 "kernel": kname, "algorithmic_bytes_per_launch": B,
                 "mean_launch_us": round(kern_s * 1e6, 2)}
     if floor is not None:
@@ -215,36 +234,10 @@ def main():
 
     # the timed precision's results (the last step's output), checked below
     res_v, res_p = dm.values.numpy(), dm.prec.numpy()
-    cpu = cpu_omp = accuracy = None
+    cpu = cpu_extra = accuracy = None
     if not args.no_cpu:
-        import pyoracle
-        t2 = time.perf_counter()
-        _, _, tus = pyoracle.fem_loop(mesh.coord, mesh.row, mesh.col, mesh.elemToNode,
-                                      mesh.checkBounds, mesh.colorToElem, op, mesh.nbNodes, 1, 0,
-                                      coef=mesh.coef)
-        cpu_s = float(np.sum(tus)) * 1e-6
-        log(f"CPU oracle REF: {cpu_s:.2f} s/iteration ({time.perf_counter() - t2:.1f}s)")
-        cpu = {"value": round(mesh.nnz / cpu_s, 1), "unit": "CSR blocks/s", "cores": 1,
-               "kind": "port",
-               "sample": f"1 full iteration (assembly + prec_init + prec_inversion, REF order, "
-                         f"serial, -O2) of the same {args.n}^3 {args.op} mesh: "
-                         f"{cpu_s:.2f} s; phases us {[round(float(x), 1) for x in tus]}",
-               "gpu_speedup": round(value / (mesh.nnz / cpu_s), 1)}
-        # the reference's OpenMP COLORING build, restated: each color's elements in parallel
-        threads = pyoracle.num_threads()
-        t3 = time.perf_counter()
-        ov, opv, tus2 = pyoracle.fem_loop(mesh.coord, mesh.row, mesh.col, mesh.elemToNode,
-                                          mesh.checkBounds, mesh.colorToElem, op, mesh.nbNodes,
-                                          1, 2, coef=mesh.coef)
-        omp_s = float(np.sum(tus2)) * 1e-6
-        log(f"CPU oracle COLORING/OpenMP ({threads} threads): {omp_s:.2f} s/iteration "
-            f"({time.perf_counter() - t3:.1f}s)")
-        cpu_omp = {"value": round(mesh.nnz / omp_s, 1), "unit": "CSR blocks/s", "cores": threads,
-                   "kind": "port",
-                   "sample": f"1 full iteration, COLORING order, elements of a color in parallel "
-                             f"(OpenMP, {threads} threads, -O2) on the same mesh: {omp_s:.3f} s",
-                   "gpu_speedup": round(value / (mesh.nnz / omp_s), 1)}
-        # the GPU results against the oracle's FP64 iteration just timed
+        cpu, cpu_extra, (ov, opv) = cpu_baselines(mesh, op, value, args)
+        # the GPU results against the oracle's FP64 iteration just timed (COLORING order)
         accuracy = {"vs": "oracle FP64 iteration, COLORING order, same mesh (the CPU leg)",
                     "values_bit_exact": bool(np.array_equal(res_v, ov)),
                     "prec_bit_exact": bool(np.array_equal(res_p, opv)),
@@ -276,11 +269,159 @@ def main():
                    "wall_s": round(wall, 4)},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "cpu_baseline_coloring_omp": cpu_omp,
+        "cpu_baselines": cpu_extra,
         "accuracy": accuracy,
         "tuning": tuning,
     }
+    if not args.no_layers and args.n == 100 and args.op == "ela" and plan is not None:
+        # the other two workloads of the headline metric, on the same GPU in the same run
+        del plan, dm
+        out["batched"] = batched_leg(args)
+        out["served"] = served_leg(args)
     print(json.dumps(out), flush=True)
+
+
+def host_cpu():
+    """Model name and the cores this process may run on (the GPU box's CPU share)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "cpus_visible": os.cpu_count(),
+            "cpus_allowed": len(os.sched_getaffinity(0))}
+
+
+def cpu_baselines(mesh, op, gpu_value, args):
+    """The host-core baselines on the workload mesh, as the reference times them
+    (src/FEM.cc:177-257: per-phase averages, iteration 0 excluded when there are more than
+    one, src/FEM.cc:182): the oracle's REF order (one thread) and its OpenMP COLORING
+    build twice, as written (serial element loop within each color, src/assembly.cc:362;
+    parallel zero-fill and prec loops) and with each color's elements in parallel. The
+    GPU speedup is quoted against the faster COLORING run (cpu_baseline) and against REF.
+    Returns (cpu_baseline, the three runs, (values, prec) of the per-color parallel run)."""
+    import numpy as np
+    import pyoracle
+    runs = {}
+    keep = None
+    threads = pyoracle.num_threads()
+    for name, mode, iters, cores in (("coloring_omp_parallel_colors", 2, args.cpu_iters, threads),
+                                     ("coloring_omp_as_written", 1, 2, threads),
+                                     ("ref", 0, 2, 1)):
+        t0 = time.perf_counter()
+        v, p, tus = pyoracle.fem_loop(mesh.coord, mesh.row, mesh.col, mesh.elemToNode,
+                                      mesh.checkBounds, mesh.colorToElem, op, mesh.nbNodes,
+                                      iters, mode, coef=mesh.coef)
+        s_it = float(np.sum(tus)) * 1e-6
+        runs[name] = {"value": round(mesh.nnz / s_it, 1), "unit": "CSR blocks/s", "cores": cores,
+                      "kind": "port", "s_per_iteration": round(s_it, 4),
+                      "phase_us": [round(float(x), 1) for x in tus],
+                      "iterations": iters, "timed_iterations": max(1, iters - 1),
+                      "gpu_speedup": round(gpu_value / (mesh.nnz / s_it), 1)}
+        log(f"CPU {name}: {s_it:.3f} s/iteration ({time.perf_counter() - t0:.1f}s, {cores} threads)")
+        if mode == 2:
+            keep = (v, p)
+        del v, p
+    best = min(("coloring_omp_parallel_colors", "coloring_omp_as_written"),
+               key=lambda k: runs[k]["s_per_iteration"])
+    b = runs[best]
+    cpu = {"value": b["value"], "unit": "CSR blocks/s", "cores": b["cores"], "kind": "port",
+           "sample": f"the oracle's OpenMP COLORING iteration ({best}, {b['cores']} threads, "
+                     f"-O2), {b['iterations']} iterations of the same {args.n}^3 {args.op} "
+                     f"mesh, iteration 0 excluded: {b['s_per_iteration']} s per iteration "
+                     f"(assembly + prec_init + prec_inversion)",
+           "gpu_speedup": b["gpu_speedup"], "gpu_speedup_vs_ref_1_core": runs["ref"]["gpu_speedup"],
+           "host": host_cpu()}
+    return cpu, runs, keep
+
+
+def batched_leg(args):
+    """The batched workload in the default line: 64 meshes (5k-200k nodes, lap and ela
+    alternating) in the union form with the library's default schedule, one captured
+    graph per batch, FP64 and mixed; every mixed mesh's error against FP64."""
+    import numpy as np
+
+    import minifem
+    from minifem.batched import Batch, batch_meshes
+    t0 = time.time()
+    meshes = batch_meshes()
+    B = sum(algorithmic_bytes(m.nbElem, m.nbNodes, m.nnz, m.operatorID) for m in meshes)
+    u = Batch(meshes, union=True)
+    out = {"workload": f"{len(meshes)} jittered cubes of 5k-200k nodes, lap/ela alternating: "
+                       f"{sum(m.nbNodes for m in meshes)} nodes, {u.blocks()} CSR blocks",
+           "form": "union (one plan per operator), captured in one HIP graph",
+           "launches_per_batch": u.launches()}
+    ref = None
+    for name, prec in (("fp64", minifem.PREC_FP64), ("mixed", minifem.PREC_MIXED)):
+        u.set_precision(prec)
+        u.capture()
+        ms = u.time_ms(iters=max(5, min(args.steps, 20)), warmup=args.warmup)
+        gbs = B / (ms * 1e-3) / 1e9
+        out[name] = {"ms_per_batch": round(ms, 4), "blocks_per_s": round(u.blocks() / (ms * 1e-3), 1),
+                     "algorithmic_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        got = u.results()
+        if ref is None:
+            ref = got
+        else:
+            out[name]["max_rel_err_vs_fp64"] = {
+                "values": max(rel_fro(v, r[0]) for (v, _), r in zip(got, ref)),
+                "prec": max(rel_fro(p, r[1]) for (_, p), r in zip(got, ref))}
+        del got
+    out["mixed_vs_fp64_throughput"] = round(out["fp64"]["ms_per_batch"] / out["mixed"]["ms_per_batch"], 3)
+    log(f"batched: {out} ({time.time() - t0:.1f}s)")
+    del u, ref
+    return out
+
+
+def served_leg(args, rate=5000.0, count=600):
+    """The served workload in the default line: 16 resident meshes (1k-885k nodes), a
+    seeded Poisson stream under an HBM budget of half the footprints' sum, served by
+    plain FP64 one request at a time and by continuous batching over 4 slots in FP64 and
+    in mixed precision. Latency runs from arrival to the end of the request's last kernel
+    (the results stay in HBM for the consumer; readback is not included)."""
+    import minifem
+    from minifem.serve import Server, make_requests, served_meshes
+    t0 = time.time()
+    meshes = served_meshes()
+    reqs = make_requests(meshes, count, rate)
+    fps = [8 * m.dim * (m.nnz + m.nbNodes) + 8 * max(m.coord.size, m.coef.size) for m in meshes]
+    budget = max(int(0.5 * sum(fps)), int(1.05 * max(fps)) + 4096)
+    out = {"workload": f"{len(meshes)} resident meshes of {min(m.nbNodes for m in meshes)}-"
+                       f"{max(m.nbNodes for m in meshes)} nodes, {count} Poisson requests at "
+                       f"{rate:g}/s (coefficient field or coordinates), HBM budget "
+                       f"{budget / 1e9:.2f} GB of {sum(fps) / 1e9:.2f} GB footprints",
+           "latency": "arrival to the end of the request's last kernel; readback excluded"}
+    sample = reqs[:8]
+    ref = {}
+    for name, prec, slots in (("plain_fp64_1_slot", minifem.PREC_FP64, 1),
+                              ("batched_fp64_4_slots", minifem.PREC_FP64, 4),
+                              ("batched_mixed_4_slots", minifem.PREC_MIXED, 4)):
+        srv = Server(meshes, budget, slots=slots, precision=prec)
+        srv.stage(reqs)
+        st = srv.run(reqs)
+        out[name] = {k: round(v, 4) for k, v in st.items()}
+        got = {r.rid: srv.run_alone(r) for r in sample}
+        if not ref:
+            ref = got
+        else:
+            out[name]["sample_max_rel_err_vs_plain_fp64"] = {
+                "requests": len(sample),
+                "values": max(rel_fro(got[k][0], ref[k][0]) for k in got),
+                "prec": max(rel_fro(got[k][1], ref[k][1]) for k in got)}
+        del srv, got
+    rps = {k: out[k]["requests_per_s"] for k in ("plain_fp64_1_slot", "batched_fp64_4_slots",
+                                                  "batched_mixed_4_slots")}
+    out["throughput_ratios"] = {
+        "batched_fp64_vs_plain": round(rps["batched_fp64_4_slots"] / rps["plain_fp64_1_slot"], 3),
+        "batched_mixed_vs_plain": round(rps["batched_mixed_4_slots"] / rps["plain_fp64_1_slot"], 3),
+        "batched_mixed_vs_batched_fp64_same_slots":
+            round(rps["batched_mixed_4_slots"] / rps["batched_fp64_4_slots"], 3)}
+    log(f"served: {out} ({time.time() - t0:.1f}s)")
+    return out
 
 
 def rel_fro(a, b):

@@ -258,13 +258,18 @@ typedef struct mf_plan_params {
     int tile_lds_bytes;   /* LDS budget of one tile: 4096..163840 (default 155000) */
     int block_threads;    /* threads per workgroup: 512 or 1024 (default 1024) */
     int max_tile_rows;    /* cap on rows per tile, <= 256 (default 254) */
+    int id_chunks;        /* > 1: bisect each of that many contiguous node-id ranges on its
+                             own, so the tiles that run together write nearby CSR rows, at
+                             the price of more halo elements per tile (default 1) */
 } mf_plan_params;
 
 /* Host arrays: elemToNode (color-permuted, 1-based, 4nE), row[N+1], col[nnz],
  * checkBounds[3N] (may be NULL), coord[3N] (may be NULL; used only to group nodes into
  * tiles by coordinate bisection, else a breadth-first node order is used). The plan
- * copies what it needs to the GPU and checks every index its kernel follows. Limit: at
- * most 64 elements per node (MF_ERR_ARG otherwise; the colored path has none). The
+ * copies what it needs to the GPU and checks every index its kernel follows. Limits: at
+ * most 128 elements per node (a mesh that mf_coloring_creation accepts has no more: the
+ * elements at one node need pairwise different colors) and 255 CSR entries per row
+ * (MF_ERR_ARG otherwise; the colored path has neither). The
  * build runs on up to 16 host threads (environment MF_PLAN_THREADS sets the count); the
  * plan does not depend on the count. */
 int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nbNodes, int nnz,

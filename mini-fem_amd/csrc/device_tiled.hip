@@ -547,6 +547,33 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                               kk[2] = {e0 & 3, e1 & 3};
                     add_pair(t, a, kk);
                 }
+            } else if (rc.x == mf::kWalkRecord) {
+                // more than kListLen elements on a row longer than a mask: walk row a's
+                // whole incidence list, taking the elements that hold c, two at a time
+                const int incOff = ri.z & 0xFFFF, val = (ri.z >> 16) & 0xFFFF;
+                int pt = -1, pa = 0, pk = 0;  // a contribution waiting for its pair
+                for (int j = 0; j < val; j++) {
+                    const int e = linc[incOff + j], te = e >> 2;
+                    const uint32_t vv = lverts[te];
+                    const int kc = (int)(vv & 0xFF) == cl ? 0
+                                   : (int)((vv >> 8) & 0xFF) == cl ? 1
+                                   : (int)((vv >> 16) & 0xFF) == cl ? 2
+                                   : (int)(vv >> 24) == cl ? 3 : -1;
+                    if (kc < 0) continue;
+                    if (pt < 0) {
+                        pt = te;
+                        pa = e & 3;
+                        pk = kc;
+                        continue;
+                    }
+                    const int t[2] = {pt, te}, a[2] = {pa, e & 3}, kk[2] = {pk, kc};
+                    add_pair(t, a, kk);
+                    pt = -1;
+                }
+                if (pt >= 0) {  // an odd count: the last one with the zero slot
+                    const int t[2] = {pt, TE}, a[2] = {pa, 0}, kk[2] = {pk, 0};
+                    add_pair(t, a, kk);
+                }
             } else {
                 // more than kListLen elements: walk the mask over row a's incidence list
                 uint64_t msk = (uint64_t)rc.y | (uint64_t)rc.z << 32;
@@ -847,7 +874,7 @@ int launch_variant(mf_plan *p, const double *coord, const double *coef, double *
 }
 
 struct Resolved {
-    int budget, block, max_rows, slot64, slot32;
+    int budget, block, max_rows, slot64, slot32, id_chunks;
 };
 
 int resolve_params(int op, const mf_plan_params *params, Resolved *r)
@@ -857,10 +884,12 @@ int resolve_params(int op, const mf_plan_params *params, Resolved *r)
     r->budget = 155000;
     r->block = 1024;
     r->max_rows = 254;
+    r->id_chunks = 1;
     if (params) {
         if (params->tile_lds_bytes > 0) r->budget = params->tile_lds_bytes;
         if (params->block_threads > 0) r->block = params->block_threads;
         if (params->max_tile_rows > 0) r->max_rows = params->max_tile_rows;
+        if (params->id_chunks > 0) r->id_chunks = params->id_chunks;
     }
     r->slot64 = mf::stage_len(ela, true) * 8;
     r->slot32 = mf::stage_len(ela, false) * 4;
@@ -875,7 +904,7 @@ int build_checked(mf::PlanHost &h, int op, int nE, int N, int nnz, const int *e2
                   const Resolved &r, int *lds64, int *lds32)
 {
     int st = mf::build_plan(h, op, nE, N, nnz, e2n, row, col, cb, coord, r.budget, r.max_rows,
-                            r.slot64, r.block);
+                            r.slot64, r.block, r.id_chunks);
     if (st != MF_OK) return st;
     st = mf::validate_plan(h, row, col, e2n, r.slot64);
     if (st != MF_OK) return st;

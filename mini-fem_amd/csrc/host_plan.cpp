@@ -392,7 +392,7 @@ struct Emitter {
                         if (v[k] - 1 == c) vc = k;
                     }
                     if (vc < 0) continue;
-                    m |= 1ull << (j - n2eI[a]);
+                    if (j - n2eI[a] < kMaskValence) m |= 1ull << (j - n2eI[a]);
                     if (cnt < kListLen) list[cnt] = (uint16_t)(elemLocal.get(n2eV[j]) << 4 | va << 2 | vc);
                     cnt++;
                 }
@@ -401,10 +401,14 @@ struct Emitter {
                     for (int z = cnt; z < kListLen; z++) list[z] = (uint16_t)(TE << 4);
                     for (int w = 0; w < 3; w++)
                         o.task_rec.push_back((uint32_t)list[2 * w] | (uint32_t)list[2 * w + 1] << 16);
-                } else {
+                } else if (val <= kMaskValence) {
                     o.task_rec.push_back(kMaskRecord);
                     o.task_rec.push_back((uint32_t)m);
                     o.task_rec.push_back((uint32_t)(m >> 32));
+                } else {
+                    o.task_rec.push_back(kWalkRecord);
+                    o.task_rec.push_back(0u);
+                    o.task_rec.push_back(0u);
                 }
                 o.task_rec.push_back((uint32_t)(ra | q << 8 | nodeLocal.get(c) << 16 | oc << 24));
             }
@@ -440,7 +444,7 @@ struct Emitter {
 
 int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, const int *row,
                const int *col, const int *checkBounds, const double *coord, int lds_budget,
-               int max_rows, int slot_bytes, int block)
+               int max_rows, int slot_bytes, int block, int id_chunks)
 {
     p = PlanHost{};
     p.op = op;
@@ -449,7 +453,8 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
     p.nnz = nnz;
     p.block = block;
     const bool ela = op == MF_OP_ELA;
-    if (nE < 0 || N < 0 || nnz < 0 || max_rows < 1 || max_rows > kMaxTileRows) return MF_ERR_ARG;
+    if (nE < 0 || N < 0 || nnz < 0 || max_rows < 1 || max_rows > kMaxTileRows || id_chunks < 1)
+        return MF_ERR_ARG;
     if (block != 512 && block != 1024) return MF_ERR_ARG;
     if (row[0] != 0 || row[N] != nnz) return MF_ERR_ARG;
     std::vector<int> rowsAll;
@@ -486,8 +491,11 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
     while ((1 << part.spawn_depth) < threads) part.spawn_depth++;
     std::vector<Leaf> leaves;
     if (coord) {
+        // rowsAll is in ascending node id: contiguous ranges of it are id ranges
         FitScratch s;
-        part.rcb(0, (int)rowsAll.size(), s, leaves, 0);
+        const int n = (int)rowsAll.size(), K = std::min(id_chunks, std::max(n, 1));
+        for (int k = 0; k < K; k++)
+            part.rcb((int)((int64_t)n * k / K), (int)((int64_t)n * (k + 1) / K), s, leaves, 0);
     } else {
         part.bfs_chunks(leaves);
     }
@@ -563,6 +571,14 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
 
 namespace {
 
+// Which positions of a row's incidence list (up to kMaxValence) hold a tile node.
+using Holds = unsigned __int128;
+static_assert(kMaxValence <= 128, "Holds has one bit per incidence entry");
+inline int popcount128(Holds x)
+{
+    return __builtin_popcountll((uint64_t)x) + __builtin_popcountll((uint64_t)(x >> 64));
+}
+
 // Claims a CSR entry or a row for one writer; false when it was claimed before.
 inline bool claim(uint8_t *flag) { return __atomic_fetch_add(flag, 1, __ATOMIC_RELAXED) == 0; }
 
@@ -575,7 +591,7 @@ struct TileValidator {
     uint8_t *hit, *rowSeen;  // shared by all tiles, claimed atomically
 
     int check(const TileHdr &h, SmallMap &localOf, std::vector<int> &cnt,
-              std::vector<uint64_t> &holds) const
+              std::vector<Holds> &holds) const
     {
         const int R = h.n_rows;
         if (R < 1 || R > kMaxTileRows || h.n_node < R || h.n_node > kMaxTileNodes ||
@@ -649,7 +665,7 @@ struct TileValidator {
                 const int t = en >> 2, va = en & 3;
                 if (t >= h.n_elem || t <= last || verts[t * 4 + va] != ra) return MF_ERR_STATE;
                 last = t;
-                for (int k = 0; k < 4; k++) holds[verts[t * 4 + k]] |= 1ull << i;
+                for (int k = 0; k < 4; k++) holds[verts[t * 4 + k]] |= (Holds)1 << i;
             }
             if (cnt[ra] != val) return MF_ERR_STATE;
             // the row's tasks: exact masks, each block and its transpose written once
@@ -660,7 +676,7 @@ struct TileValidator {
                 const int c = col[row[a] + oa] - 1;
                 if (c == a || !claim(hit + row[a] + oa)) return MF_ERR_STATE;
                 const int ec = localOf.get(c);
-                const int nHold = ec < 0 ? 0 : __builtin_popcountll(holds[ec]);
+                const int nHold = ec < 0 ? 0 : popcount128(holds[ec]);
                 if (ec < 0 || cl != ec || nHold < 1) return MF_ERR_STATE;
                 if (nHold <= kListLen) {
                     // the exact ascending list of (element, slot of a, slot of c), padded
@@ -680,9 +696,11 @@ struct TileValidator {
                     for (; q < kListLen; q++)
                         if (((rec[q / 2] >> (16 * (q % 2))) & 0xFFFF) != (uint32_t)(h.n_elem << 4))
                             return MF_ERR_STATE;
-                } else {
+                } else if (val <= kMaskValence) {
                     const uint64_t m = (uint64_t)rec[1] | (uint64_t)rec[2] << 32;
-                    if (rec[0] != kMaskRecord || m != holds[ec]) return MF_ERR_STATE;
+                    if (rec[0] != kMaskRecord || (Holds)m != holds[ec]) return MF_ERR_STATE;
+                } else if (rec[0] != kWalkRecord || rec[1] != 0 || rec[2] != 0) {
+                    return MF_ERR_STATE;  // the kernel finds the elements on row a's list
                 }
                 if (ec < R) {  // c is a row of this tile: the task also writes (c, a)
                     if (oc >= (p.tile_rowinfo[h.rinfo_start + ec].len_diag & 0xFFFF) ||
@@ -725,7 +743,7 @@ int validate_plan(const PlanHost &p, const int *row, const int *col, const int *
         try {
             SmallMap localOf(10);
             std::vector<int> cnt(kMaxTileNodes);
-            std::vector<uint64_t> holds(kMaxTileNodes, 0);
+            std::vector<Holds> holds(kMaxTileNodes, 0);
             const int t1 = (int)((int64_t)T * (c + 1) / nc);
             for (int t = (int)((int64_t)T * c / nc); t < t1 && !stop; t++) {
                 const int st = tv.check(p.tiles[t], localOf, cnt, holds);

@@ -48,12 +48,17 @@ constexpr int kMaxTileElems = 1024;      // staged elements per tile (the zero s
 constexpr int kMaxTileNodes = 256;       // tile-local node ids are 8-bit
 constexpr int kMaxTileRows = 256;        // tile-local row ids are 8-bit
 constexpr int kMaxTileInc = 4096;        // incidence entries per tile
-constexpr int kMaxValence = 64;          // incident elements per row (mask width)
+// Incident elements per row. The elements holding one node share it, so greedy coloring
+// gives them pairwise different colors: a mesh that colors within 128 colors
+// (src/coloring.cc:22) has at most 128 per node.
+constexpr int kMaxValence = 128;
+constexpr int kMaskValence = 64;         // rows up to this valence: mask records (64 bits)
 constexpr int kMaxLdsBytes = 160 * 1024; // LDS of one gfx950 CU
 constexpr int kIncAlign = 4;             // row incidence lists start at multiples of 4
 constexpr int kXcds = 8;                 // MI355X: workgroups are dealt to 8 XCDs round-robin
 constexpr int kListLen = 6;              // contributions held in one task record
 constexpr uint32_t kMaskRecord = 0xFFFFFFFFu;  // first word of a mask task record
+constexpr uint32_t kWalkRecord = 0xFFFFFFFEu;  // first word of a walk task record
 
 // Waves of a workgroup of `block` threads (512 or 1024): loaders, then compute waves.
 MF_HOST_DEVICE constexpr int loader_waves(int block) { return block >= 1024 ? 4 : 2; }
@@ -172,9 +177,11 @@ struct RowInfo {
 // record each (task_rec, 4 words), for the block (a,c) of tile row a and column c:
 //   words 0-2: the n elements holding both a and c. n <= kListLen: six u16 contributions
 //              t << 4 | a << 2 | k (tile element, slots of a and c in it) ascending, the
-//              unused ones t = n_elem (the zero slot). n > kListLen: word 0 = kMaskRecord,
-//              words 1-2 a 64-bit mask, bit b set when the b-th element of row a's
-//              incidence list holds c.
+//              unused ones t = n_elem (the zero slot). n > kListLen and row a's valence
+//              <= kMaskValence: word 0 = kMaskRecord, words 1-2 a 64-bit mask, bit b set
+//              when the b-th element of row a's incidence list holds c. n > kListLen on a
+//              longer row: word 0 = kWalkRecord, words 1-2 zero; the lane walks row a's
+//              whole incidence list and takes the elements whose vertices include c.
 //   word 3:    ra | oa << 8 | cl << 16 | oc << 24: a's tile row, the block's offset in
 //              a's row, c's tile-local node, and (when cl < n_rows: c is a row of the same
 //              tile, and the task also writes the transpose (c,a)) the offset of (c,a) in
@@ -197,9 +204,12 @@ struct PlanHost {
 // LDS bytes of one element-stage slot of the widest variant that will run it; tiles are
 // grown while their LDS layout stays <= lds_budget and the kernel's per-thread limits
 // for a workgroup of `block` threads hold. checkBounds (3N, ela) may be null.
+// id_chunks > 1: the rows are first cut into that many contiguous node-id ranges, and
+// each range is bisected on its own (tiles then hold rows of one id range, so the tiles
+// that run at the same time write nearby CSR rows).
 int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, const int *row,
                const int *col, const int *checkBounds, const double *coord, int lds_budget,
-               int max_rows, int slot_bytes, int block);
+               int max_rows, int slot_bytes, int block, int id_chunks = 1);
 
 // Checks every index the kernel will follow against the array it indexes, that every
 // CSR entry is written exactly once (diagonal, task or transpose) and that every task's

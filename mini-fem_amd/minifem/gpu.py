@@ -327,14 +327,15 @@ class Plan:
     (mf_plan_create). assemble() runs one fused iteration: values + inverted prec."""
 
     def __init__(self, mesh, tile_lds_bytes=0, block_threads=0, max_tile_rows=0,
-                 use_coord=True, partition_coord=None):
+                 use_coord=True, partition_coord=None, id_chunks=0):
         """partition_coord: coordinates used only to group nodes into tiles (default: the
         mesh's own)."""
         from ._lib import PlanParams
         _ensure()
         self._lib = lib()
         self.mesh = mesh
-        params = PlanParams(int(tile_lds_bytes), int(block_threads), int(max_tile_rows))
+        params = PlanParams(int(tile_lds_bytes), int(block_threads), int(max_tile_rows),
+                            int(id_chunks))
         pc = None
         if use_coord:
             pc = mesh.coord if partition_coord is None else np.ascontiguousarray(

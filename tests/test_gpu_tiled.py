@@ -51,6 +51,7 @@ def test_tiled_bit_exact_30_cube(mf, orc, gpu, op, coef_seed):
     dict(tile_lds_bytes=150 * 1024, max_tile_rows=254), dict(use_coord=False),
     dict(max_tile_rows=1), dict(tile_lds_bytes=8192, block_threads=1024),
     dict(tile_lds_bytes=150 * 1024, block_threads=1024, max_tile_rows=254),
+    dict(id_chunks=8), dict(id_chunks=5, block_threads=512),
 ])
 def test_tiled_schedule_variants(mf, orc, gpu, op, params):
     print(f"[tiled variant] op={op} {params}", flush=True)
@@ -79,10 +80,12 @@ def fan_mesh(mf, orc, n_fan, op, coef_seed=5):
 
 
 @pytest.mark.parametrize("op", [0, 1])
-@pytest.mark.parametrize("n_fan", [24, 40, 64])
+@pytest.mark.parametrize("n_fan", [24, 40, 64, 65, 96, 127])
 def test_tiled_high_valence_masks(mf, orc, gpu, op, n_fan):
-    """Rows of valence 24, 40 and 64: tasks with more than six contributions take the
-    record's 64-bit mask over the row's incidence list; bit-exact."""
+    """Rows of valence 24 to 127: tasks with more than six contributions take the
+    record's 64-bit mask over the row's incidence list up to valence 64, and walk the
+    row's whole list beyond; bit-exact. 127 tets at one edge need 127 colors, the most a
+    node can have in a mesh that colors within 128."""
     print(f"[tiled fan] op={op} n_fan={n_fan}", flush=True)
     mesh, om = fan_mesh(mf, orc, n_fan, op)
     plan, dm = run_plan(gpu, mesh)

@@ -12,7 +12,7 @@ from minifem.setup import plan_host_stats
 @pytest.mark.parametrize("params", [
     dict(), dict(tile_lds_bytes=8192), dict(tile_lds_bytes=150 * 1024, max_tile_rows=254),
     dict(max_tile_rows=1), dict(use_coord=False), dict(block_threads=512),
-    dict(block_threads=1024),
+    dict(block_threads=1024), dict(id_chunks=3), dict(id_chunks=8, tile_lds_bytes=40000),
 ])
 def test_plan_builds_and_validates(mf, n, shuffle, op, params):
     coord, e2n, code, nnz = mf.generate_cube(n, 0.2, 3, shuffle)
@@ -53,25 +53,6 @@ def test_plan_rejects_inconsistent_pattern(mf):
         plan_host_stats(mesh)
 
 
-def test_plan_rejects_valence_above_64(mf):
-    """A node in more than 64 elements does not fit the 64-bit entry masks: the plan is
-    refused (the colored schedule handles such meshes)."""
-    # a fan of 70 tets around the edge (0, 1): nodes 0, 1 in every element
-    n_fan = 70
-    ang = np.linspace(0, 2 * np.pi, n_fan + 1)[:-1]
-    ring = np.stack([np.cos(ang), np.sin(ang), np.full(n_fan, 0.5)], 1)
-    coord = np.concatenate([[[0, 0, 0], [0, 0, 1]], ring]).astype(np.float64)
-    e2n = []
-    for i in range(n_fan):
-        e2n += [1, 2, 3 + i, 3 + (i + 1) % n_fan]
-    e2n = np.array(e2n, np.int32)
-    code = np.zeros(coord.shape[0], np.int32)
-    nnz = 2 * (n_fan + 2) + 5 * n_fan  # hub rows see every node, ring rows 5
-    mesh = mf.Mesh(coord.ravel(), e2n, code, nnz, 0)
-    with pytest.raises(mf.MiniFemError):
-        plan_host_stats(mesh)
-
-
 def test_plan_accepts_valence_between_32_and_64(mf):
     """Valence 33..64 selects the two-word entry masks."""
     n_fan = 40
@@ -101,3 +82,47 @@ def test_plan_independent_of_host_threads(mf, monkeypatch, use_coord):
         got.append(plan_host_stats(mesh, use_coord=use_coord))
     assert got[0]["tiles"] >= 128  # enough tiles for several emission chunks
     assert got[0] == got[1]
+
+
+def fan(mf, n_fan, op):
+    """n_fan tets around the edge between two hub nodes (valence n_fan at both hubs)."""
+    ang = np.linspace(0, 2 * np.pi, n_fan + 1)[:-1]
+    ring = np.stack([np.cos(ang), np.sin(ang), 0.5 + 0.1 * np.sin(3 * ang)], 1)
+    coord = np.concatenate([[[0, 0, 0], [0.05, -0.02, 1]], ring]).astype(np.float64).ravel()
+    e2n = np.array([[1, 2, 3 + i, 3 + (i + 1) % n_fan] for i in range(n_fan)], np.int32).ravel()
+    code = np.zeros(n_fan + 2, np.int32)
+    return mf.Mesh(coord, e2n, code, 2 * (n_fan + 2) + 5 * n_fan, op)
+
+
+@pytest.mark.parametrize("n_fan", [64, 65, 96, 127])
+@pytest.mark.parametrize("op", [0, 1])
+def test_plan_high_valence(mf, n_fan, op):
+    """Rows with more incident elements than a mask has bits (64) build and validate: the
+    hub-hub task takes a walk record; 127 tets at one edge need 127 colors, the most a
+    mesh can have at a node and still color within 128."""
+    mesh = fan(mf, n_fan, op)
+    assert mesh.nbColors == n_fan
+    s = plan_host_stats(mesh)
+    assert s["tiles"] >= 1 and s["tile_elems"] == n_fan
+
+
+def test_plan_rejects_valence_above_128(mf):
+    """129 elements at one node cannot be colored within 128 colors, and the plan refuses
+    them too (the mesh is built without coloring: elements in file order)."""
+    n_fan = 129
+    ang = np.linspace(0, 2 * np.pi, n_fan + 1)[:-1]
+    ring = np.stack([np.cos(ang), np.sin(ang), 0.5 + 0.1 * np.sin(3 * ang)], 1)
+    coord = np.concatenate([[[0, 0, 0], [0.05, -0.02, 1]], ring]).astype(np.float64).ravel()
+    e2n = np.array([[1, 2, 3 + i, 3 + (i + 1) % n_fan] for i in range(n_fan)], np.int32).ravel()
+    with pytest.raises(mf.MiniFemError):
+        mf.coloring_creation(e2n, n_fan + 2)
+    row, col = mf.create_nodeToNode(e2n, n_fan + 2, 2 * (n_fan + 2) + 5 * n_fan)
+    nnz = col.size
+
+    class M:  # the fields plan_host_stats reads
+        pass
+    m = M()
+    m.operatorID, m.nbElem, m.nbNodes, m.nnz = 1, n_fan, n_fan + 2, nnz
+    m.elemToNode, m.row, m.col, m.coord = e2n, row, col, coord
+    with pytest.raises(mf.MiniFemError):
+        plan_host_stats(m)
