@@ -12,6 +12,7 @@
 #include <thread>
 #include <vector>
 
+#include "host_threads.h"
 #include "minifem.h"
 #include "plan.h"
 
@@ -19,15 +20,7 @@ namespace mf {
 
 namespace {
 
-int plan_threads()
-{
-    if (const char *e = std::getenv("MF_PLAN_THREADS")) {
-        const int n = std::atoi(e);
-        if (n >= 1) return std::min(n, 64);
-    }
-    const unsigned h = std::thread::hardware_concurrency();
-    return std::max(1, std::min(16, (int)h));
-}
+int plan_threads() { return host_threads(); }
 
 // Map from non-negative ids to ints for the ids of one tile (a few hundred nodes, up to
 // ~1K elements), open addressing, cleared in O(1) by a generation counter. A map that
@@ -464,23 +457,13 @@ int build_plan(PlanHost &p, int op, int nE, int N, int nnz, const int *e2n, cons
         if (len == 0) p.isolated.push_back(i);
         else rowsAll.push_back(i);
     }
+    // each node's elements, ascending (the threaded counting fill of the setup)
     std::vector<int> n2eI((size_t)N + 1, 0), n2eV((size_t)nE * 4 + 1);
-    for (int64_t q = 0; q < (int64_t)nE * 4; q++) {
-        const int v = e2n[q] - 1;
-        if (v < 0 || v >= N) return MF_ERR_ARG;
-        n2eI[v + 1]++;
-    }
+    const int st = mf_create_nodeToElem(e2n, nE, N, n2eI.data(), n2eV.data());
+    if (st != MF_OK) return st;
     int maxVal = 0;
-    for (int i = 0; i < N; i++) {
-        maxVal = std::max(maxVal, n2eI[i + 1]);
-        n2eI[i + 1] += n2eI[i];
-    }
+    for (int i = 0; i < N; i++) maxVal = std::max(maxVal, n2eI[i + 1] - n2eI[i]);
     if (maxVal > kMaxValence) return MF_ERR_ARG;
-    {
-        std::vector<int> cur(n2eI.begin(), n2eI.end() - 1);
-        for (int e = 0; e < nE; e++)
-            for (int k = 0; k < 4; k++) n2eV[cur[e2n[(int64_t)e * 4 + k] - 1]++] = e;
-    }
     // a node with elements has a non-empty row holding its diagonal
     for (int i = 0; i < N; i++)
         if ((n2eI[i + 1] > n2eI[i]) != (row[i + 1] > row[i])) return MF_ERR_ARG;

@@ -176,3 +176,42 @@ def test_too_many_colors(mf):
 def test_host_norm(mf, orc):
     x = np.random.default_rng(1).standard_normal(5000)
     assert abs(mf.compute_double_norm(x) - orc.norm(x)) <= 1e-14 * orc.norm(x)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_coloring_any_element_order(mf, orc, seed):
+    """Greedy first fit depends on the element order: random element orders of a node-
+    shuffled cube, the product's per-node color masks against the oracle's walk over
+    every element sharing a node (src/coloring.cc:46-81), bit for bit."""
+    coord, e2n, code, nnz = mf.generate_cube(14, 0.2, seed, 100 + seed)
+    order = np.random.default_rng(seed).permutation(e2n.size // 4)
+    e2n = np.ascontiguousarray(e2n.reshape(-1, 4)[order].ravel())
+    nN = coord.size // 3
+    got, want = mf.coloring_creation(e2n, nN), orc.coloring(e2n, nN)
+    assert all(np.array_equal(a, b) for a, b in zip(got, want))
+    assert len(got[2]) - 1 > 20  # a random order needs many more colors than the cube's 30s
+
+
+def test_coloring_60_cube_shuffled(mf, orc):
+    coord, e2n, code, nnz = mf.generate_cube(60, 0.25, 3, 203)
+    nN = coord.size // 3
+    got, want = mf.coloring_creation(e2n, nN), orc.coloring(e2n, nN)
+    assert all(np.array_equal(a, b) for a, b in zip(got, want))
+
+
+@pytest.mark.parametrize("threads", ["1", "3", "8"])
+def test_setup_independent_of_host_threads(mf, orc, monkeypatch, threads):
+    """nodeToElem and the CSR pattern run on several host threads (chunks of elements,
+    ranges of rows); the result is the oracle's, whatever the count."""
+    monkeypatch.setenv("MF_PLAN_THREADS", threads)
+    coord, e2n, code, nnz = mf.generate_cube(40, 0.25, 3, 203)
+    nN = coord.size // 3
+    i1, v1 = mf.create_nodeToElem(e2n, nN)
+    i2, v2 = orc.nodeToElem(e2n, nN)
+    assert np.array_equal(i1, i2) and np.array_equal(v1, v2)
+    r1, c1 = mf.create_nodeToNode(e2n, nN, nnz, n2e=(i1, v1))
+    r2, c2 = orc.csr(e2n, nN, nnz)
+    assert np.array_equal(r1, r2) and np.array_equal(c1, c2)
+    with pytest.raises(mf.MiniFemError) as ei:  # capacity one short
+        mf.create_nodeToNode(e2n, nN, nnz - 1, n2e=(i1, v1))
+    assert ei.value.status == -4
