@@ -240,6 +240,16 @@ int mf_pcg(int operatorID, int nbNodes, const int *row, const int *col, const do
  * squares, sqrt on the host. x is device memory; blocks until the result is ready. */
 int mf_norm(const double *x, int64_t n, double *result, void *stream);
 
+/* mf_norm without any device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same kernels and summation order as mf_norm,
+ * so the same result bit for bit. Its scratch is a fixed 8 KiB array in device memory (the
+ * 1024 per-workgroup partial sums), one per GPU, held until the process ends; any n >= 0 is
+ * accepted, since the partial sums do not grow with n. Every call on one GPU shares that
+ * GPU's scratch: calls that overlap on one GPU (several streams or host threads at once)
+ * race and must be serialized by the caller. Blocks until the result is ready. */
+int mf_norm_noalloc(const double *x, int64_t n, double *result, void *stream);
+
 /* ---------------------------------------------------------------------------------
  * Plans: a mesh resident on the GPU with a precomputed tiled gather schedule.
  * One plan = one mesh topology (color-permuted elemToNode, CSR pattern, boundary

@@ -11,6 +11,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "device_common.h"
@@ -206,6 +207,36 @@ int launch_colors(const double *coord, const double *coef, double *values, const
 
 }  // namespace
 
+// Scratch of the allocation-free norm (mf_norm_noalloc): the kNormBlocks per-workgroup
+// partial sums of k_sumsq_partial, folded in place by k_sumsq_final. 8 KiB, held by
+// every GPU the library runs on until the process ends, each GPU its own copy at its own
+// address. Host code never passes `scratch` itself (a host-side stand-in): it uses the
+// per-device address resolved by hipGetSymbolAddress below.
+static __device__ __align__(256) double scratch[kNormBlocks];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device `dev`, resolved once per device
+// (at mf_device_init, or on the first allocation-free call on a device that skipped it).
+int norm_scratch(int dev, double **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev < 0 || dev >= kMaxDevices) return MF_ERR_HIP;
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MF_ERR_HIP;
+        g_scratch[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch[dev];
+    return MF_OK;
+}
+
+}  // namespace
+
 extern "C" const char *mf_status_string(int status)
 {
     switch (status) {
@@ -243,7 +274,8 @@ extern "C" int mf_device_init(int device)
                      device, prop.gcnArchName);
         return MF_ERR_HIP;
     }
-    return MF_OK;
+    double *unused = nullptr;
+    return norm_scratch(device, &unused);
 }
 
 extern "C" int mf_assembly_ex(const double *coord, const double *coef, double *values,
@@ -349,6 +381,29 @@ extern "C" int mf_norm(const double *x, int64_t n, double *result, void *stream)
     hipError_t e3 = hipFreeAsync(part, s);
     hipError_t e4 = hipStreamSynchronize(s);
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess)
+        return MF_ERR_HIP;
+    *result = __builtin_sqrt(h);
+    return MF_OK;
+}
+
+extern "C" int mf_norm_noalloc(const double *x, int64_t n, double *result, void *stream)
+{
+    if (n < 0 || !result || (n > 0 && !x)) return MF_ERR_ARG;
+    if (n == 0) { *result = 0.0; return MF_OK; }
+    int dev = 0;
+    double *part = nullptr;
+    if (hipGetDevice(&dev) != hipSuccess) return MF_ERR_HIP;
+    const int st = norm_scratch(dev, &part);
+    if (st != MF_OK) return st;
+    // the same two kernels and order as mf_norm: every workgroup writes its own partial
+    // (zero when it sums nothing), so nothing in the scratch has to be cleared first
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_sumsq_partial, dim3(kNormBlocks), dim3(kBlock), 0, s, x, n, part);
+    hipLaunchKernelGGL(k_sumsq_final, dim3(1), dim3(kBlock), 0, s, part);
+    double h = 0.0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&h, part, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
         return MF_ERR_HIP;
     *result = __builtin_sqrt(h);
     return MF_OK;

@@ -59,6 +59,7 @@ SIGNATURES = {
     "mf_prec_init": (_i, [_p, _p, _p, _p, _i, _i, _p]),
     "mf_prec_inversion": (_i, [_p, _p, _p, _p, _i, _i, _p, _p]),
     "mf_norm": (_i, [_p, _i64, _p, _p]),
+    "mf_norm_noalloc": (_i, [_p, _i64, _p, _p]),
     "mf_dev_alloc": (_i, [_p, ctypes.c_size_t]),
     "mf_dev_free": (_i, [_p]),
     "mf_memcpy_h2d": (_i, [_p, _p, ctypes.c_size_t, _p]),
