@@ -83,6 +83,10 @@ def main():
     ap.add_argument("--write-probe", action="store_true",
                     help="instead of the FEM step, time the step's output writes alone: the "
                          "tile rows in the kernel's order vs the same bytes front to back")
+    ap.add_argument("--gather", type=int, default=0,
+                    help="instead of the FEM step, time the random gather probe: this many "
+                         "8-byte reads at seeded random positions of a 1 GiB array (the byte "
+                         "counters' scale for scattered reads)")
     ap.add_argument("--copy", type=int, default=0,
                     help="instead of the FEM step, time the plain HBM copy kernel on this many "
                          "bytes (bandwidth reference and byte-counter calibration)")
@@ -114,6 +118,32 @@ def main():
         ms = e0.elapsed_ms(e1) / args.steps
         print(json.dumps({"copy_bytes": n, "ms_per_copy": ms,
                           "gbs_read_plus_write": 2 * n / ms / 1e6}), flush=True)
+        return
+
+    if args.gather:
+        gpu.init(0)
+        span = 1 << 27  # doubles: 1 GiB
+        n = int(args.gather)
+        rng = np.random.default_rng(7)
+        idx = rng.integers(0, span, n, dtype=np.int64)
+        src = gpu.DeviceArray(span, np.float64)
+        didx = gpu.DeviceArray.from_numpy(idx)
+        dst = gpu.DeviceArray(n, np.float64)
+        stream = gpu.Stream()
+        lib = minifem.lib()
+        for _ in range(args.warmup):
+            minifem.check(lib.mf_gather_probe(dst.ptr, src.ptr, didx.ptr, n, stream.ptr), "gather")
+        e0, e1 = gpu.Event(), gpu.Event()
+        e0.record(stream)
+        for _ in range(args.steps):
+            minifem.check(lib.mf_gather_probe(dst.ptr, src.ptr, didx.ptr, n, stream.ptr), "gather")
+        e1.record(stream)
+        ms = e0.elapsed_ms(e1) / args.steps
+        # distinct 64- and 128-byte lines the gathers touch (what a cold DRAM read moves)
+        lines = {w: int(np.unique(idx // (w // 8)).size) for w in (32, 64, 128)}
+        print(json.dumps({"gathers": n, "ms": ms, "index_bytes": 8 * n, "dst_bytes": 8 * n,
+                          "distinct_lines": lines,
+                          "gather_bytes_if_line": {w: lines[w] * w for w in lines}}), flush=True)
         return
 
     if args.batched:
@@ -377,14 +407,16 @@ def batched_leg(args):
     return out
 
 
-def served_leg(args, rate=5000.0, count=600):
+def served_leg(args, rate=5000.0, count=600, saturating=100000.0):
     """The served workload in the default line: 16 resident meshes (1k-885k nodes), a
     seeded Poisson stream under an HBM budget of half the footprints' sum, served by
     plain FP64 one request at a time and by continuous batching over 4 slots in FP64 and
-    in mixed precision. Latency runs from arrival to the end of the request's last kernel
-    (the results stay in HBM for the consumer; readback is not included)."""
+    in mixed precision. Each server takes the stream at `rate` (latency) and the same
+    requests at a saturating rate (throughput: every form busy all the time). Latency
+    runs from arrival to the end of the request's last kernel (the results stay in HBM for
+    the consumer; readback is not included)."""
     import minifem
-    from minifem.serve import Server, make_requests, served_meshes
+    from minifem.serve import Server, make_requests, retime, served_meshes
     t0 = time.time()
     meshes = served_meshes()
     reqs = make_requests(meshes, count, rate)
@@ -400,10 +432,15 @@ def served_leg(args, rate=5000.0, count=600):
     for name, prec, slots in (("plain_fp64_1_slot", minifem.PREC_FP64, 1),
                               ("batched_fp64_4_slots", minifem.PREC_FP64, 4),
                               ("batched_mixed_4_slots", minifem.PREC_MIXED, 4)):
-        srv = Server(meshes, budget, slots=slots, precision=prec)
+        # plain: one request per launch, one launch at a time; batched: up to 8 requests
+        # per launch on each of 4 streams
+        srv = Server(meshes, budget, slots=slots, precision=prec, max_group=1 if slots == 1 else 8)
         srv.stage(reqs)
         st = srv.run(reqs)
         out[name] = {k: round(v, 4) for k, v in st.items()}
+        sat = srv.run(retime(reqs, saturating))
+        out[name]["saturated"] = {k: round(v, 4) for k, v in sat.items()
+                                  if k in ("p50_ms", "p99_ms", "requests_per_s", "blocks_per_s")}
         got = {r.rid: srv.run_alone(r) for r in sample}
         if not ref:
             ref = got
@@ -412,10 +449,28 @@ def served_leg(args, rate=5000.0, count=600):
                 "requests": len(sample),
                 "values": max(rel_fro(got[k][0], ref[k][0]) for k in got),
                 "prec": max(rel_fro(got[k][1], ref[k][1]) for k in got)}
-        del srv, got
-    rps = {k: out[k]["requests_per_s"] for k in ("plain_fp64_1_slot", "batched_fp64_4_slots",
-                                                  "batched_mixed_4_slots")}
-    out["throughput_ratios"] = {
+        del got
+        if name != "batched_mixed_4_slots":  # the last server also admits a new mesh
+            del srv
+    # a topology that was not resident: host setup (coloring, CSR, mask), then the plan,
+    # device arrays and a warm launch, while the server holds its 16 meshes
+    from minifem.setup import Mesh, generate_coef, generate_cube
+    coord, e2n, code, nnz = generate_cube(100, 0.2, 4242, 4243)
+    coef = generate_coef(1, e2n.size // 4, 4244)
+    t1 = time.perf_counter()
+    mesh = Mesh(coord, e2n, code, nnz, 1, coef=coef)
+    t2 = time.perf_counter()
+    idx = srv.add_mesh(mesh)
+    t3 = time.perf_counter()
+    out["new_mesh_admission"] = {"mesh": f"ela 100^3 node-shuffled, {mesh.nbNodes} nodes, "
+                                         f"{mesh.nbElem} tets", "host_setup_s": round(t2 - t1, 3),
+                                 "plan_upload_warm_s": round(t3 - t2, 3),
+                                 "total_s": round(t3 - t1, 3), "mesh_index": idx}
+    del srv, mesh
+    rps = {k: out[k]["saturated"]["requests_per_s"]
+           for k in ("plain_fp64_1_slot", "batched_fp64_4_slots", "batched_mixed_4_slots")}
+    out["saturated_rate_hz"] = saturating
+    out["throughput_ratios"] = {  # at the saturating rate
         "batched_fp64_vs_plain": round(rps["batched_fp64_4_slots"] / rps["plain_fp64_1_slot"], 3),
         "batched_mixed_vs_plain": round(rps["batched_mixed_4_slots"] / rps["plain_fp64_1_slot"], 3),
         "batched_mixed_vs_batched_fp64_same_slots":
@@ -522,7 +577,7 @@ def bench_serve(args):
     for name, prec, slots in (("plain_fp64_one_at_a_time", minifem.PREC_FP64, 1),
                               ("batched_fp64", minifem.PREC_FP64, 4),
                               ("batched_mixed", minifem.PREC_MIXED, 4)):
-        srv = Server(meshes, budget, slots=slots, precision=prec)
+        srv = Server(meshes, budget, slots=slots, precision=prec, max_group=1 if slots == 1 else 8)
         srv.stage(reqs)
         st = srv.run(reqs)
         out[name] = {k: round(v, 4) for k, v in st.items()}

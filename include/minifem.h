@@ -194,6 +194,11 @@ int mf_graph_destroy(void *graph);
  * 16): the HBM bandwidth reference for rooflines and byte-counter calibration. */
 int mf_hbm_copy(void *dst, const void *src, size_t bytes, void *stream);
 
+/* Measurement only: dst[i] = src[idx[i]] for i < n (device arrays; every idx in range is
+ * the caller's duty). Random 8-byte reads move whole DRAM sectors: the reference for the
+ * byte counters' scale on scattered reads. */
+int mf_gather_probe(double *dst, const double *src, const int64_t *idx, int64_t n, void *stream);
+
 /* assembly (src/headers/assembly.h:61-74, called at src/FEM.cc:183), COLORING
  * semantics (src/assembly.cc:593-611, :660-673): zero values[nbEdges*operatorDim],
  * then one race-free launch per color over elements [colorToElem[c], colorToElem[c+1])
@@ -310,6 +315,38 @@ int mf_plan_stats(const mf_plan *plan, int64_t out[8]);
 int mf_plan_assemble(mf_plan *plan, const double *coord, const double *coef,
                      double *values, double *prec, int precision, int *singular,
                      void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * Several plans in one launch (no counterpart in the reference, which assembles one
+ * mesh: this is the single launch of the batched and served forms).
+ * ------------------------------------------------------------------------------- */
+
+#define MF_MAX_PLAN_JOBS 16
+
+/* One plan's iteration in a multi-plan launch: what mf_plan_assemble takes for it. */
+typedef struct mf_plan_job {
+    const mf_plan *plan;
+    const double *coord;   /* device, 3N */
+    const double *coef;    /* device, NULL = reference constants (float with F32COEF) */
+    double *values;        /* device, nnz*d */
+    double *prec;          /* device, N*d */
+    int *singular;         /* device, may be NULL */
+} mf_plan_job;
+
+/* The iterations of 1..MF_MAX_PLAN_JOBS plans, lap and ela mixed, in ONE persistent
+ * launch: the tiles of every job, one job after another, dealt to the workgroups as one
+ * plan's tiles are. Each job's values and prec are bit-identical to its own
+ * mf_plan_assemble. The plans must share block_threads, and their largest tiles must fit
+ * one CU's LDS together with a front part sized for both operators (MF_ERR_ARG otherwise:
+ * launch them in smaller groups; mf_plans_lds_bytes tells in advance). The jobs' outputs
+ * must not overlap. A plan with nodes outside every element adds one small launch for
+ * their prec. The job table travels in the kernel arguments, so the call can be captured
+ * into a graph and the caller's table can be reused at once. */
+int mf_plans_assemble(const mf_plan_job *jobs, int njobs, int precision, void *stream);
+
+/* LDS bytes per workgroup that mf_plans_assemble would use for these jobs (0 when they
+ * have no tiles), or a negative status (MF_ERR_ARG also when they do not fit one CU). */
+int mf_plans_lds_bytes(const mf_plan_job *jobs, int njobs, int precision);
 
 /* Measurement only, no counterpart in the reference: writes zeros over values[nnz*d]
  * with the assembly's output stream alone, to time what HBM accepts for it. mode 0:

@@ -28,7 +28,28 @@ __global__ void __launch_bounds__(256)
     for (; i < n16; i += stride) dst[i] = src[i];
 }
 
+// Random gather of 8-byte values: dst[i] = src[idx[i]]. Measurement only: the byte
+// counters' scale for scattered reads, which move whole DRAM sectors per value.
+__global__ void __launch_bounds__(256)
+    k_gather(double *__restrict__ dst, const double *__restrict__ src,
+             const int64_t *__restrict__ idx, int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) dst[i] = src[idx[i]];
+}
+
 }  // namespace
+
+extern "C" int mf_gather_probe(double *dst, const double *src, const int64_t *idx, int64_t n,
+                               void *stream)
+{
+    if (n < 0 || (n > 0 && (!dst || !src || !idx))) return MF_ERR_ARG;
+    if (n == 0) return MF_OK;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 16);
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)blocks), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), dst, src, idx, n);
+    return hipGetLastError() == hipSuccess ? MF_OK : MF_ERR_HIP;
+}
 
 extern "C" int mf_hbm_copy(void *dst, const void *src, size_t bytes, void *stream)
 {

@@ -6,7 +6,9 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -21,6 +23,7 @@ struct mf_plan {
     int ntiles = 0, block = 512;
     int max_node = 0, max_elem = 0;  // the largest tile: sizes the fixed front of its LDS
     int lds_fp64 = 0, lds_mixed = 0;
+    int back_fp64 = 0, back_mixed = 0;  // the largest tile's LDS past the fixed front part
     int n_isolated = 0;
     int grid[3] = {0, 0, 0};  // persistent grid per precision mode, set at creation
     int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -98,10 +101,495 @@ __device__ __forceinline__ void ela_vertex_store(float *p, const double (&g)[3],
     *reinterpret_cast<float4 *>(p) = make_float4((float)g[0], (float)g[1], (float)g[2], (float)cd);
 }
 
-// Persistent workgroups, XCD-aware. The tiles are cut into 8 contiguous ranges in
-// partition order, range x for XCD x (workgroup b runs on XCD b % 8); the G workgroups of
-// an XCD walk its range together, slot s0, s0 + G, ..., so tiles that share elements and
-// nodes run on one XCD close in time and share its L2.
+constexpr int kOpsAny = -1;  // operator per job, read from the job at run time
+
+// One plan's launch: its schedule arrays, the buffers its iteration reads and writes, and
+// its tile count.
+struct TileJob {
+    const mf::TileHdr *tiles;
+    const int32_t *tile_elems;
+    const uint32_t *tile_verts;
+    const int32_t *tile_nodes;
+    const int4 *rowinfo;
+    const uint4 *inc8;
+    const uint4 *task_rec;
+    const double *coord;
+    const double *coef;
+    double *values;
+    double *prec;
+    int *singular;
+    int op, ntiles, pad0, pad1;
+};
+
+// Several plans in one launch, passed by value in the kernel arguments (no device table
+// to upload; a captured graph keeps its own copy).
+constexpr int kMaxJobs = MF_MAX_PLAN_JOBS;
+struct JobTable {
+    TileJob job[kMaxJobs];
+    int njobs;
+};
+
+// The jobs of a launch as the kernel body sees them.
+struct OneJob {
+    const TileJob &j;
+    __device__ int count() const { return 1; }
+    __device__ const TileJob &at(int) const { return j; }
+};
+struct ManyJobs {
+    const JobTable &t;
+    __device__ int count() const { return t.njobs; }
+    __device__ const TileJob &at(int i) const { return t.job[i]; }
+};
+
+// Tile header i, read through the constant address space: a uniform load there is a
+// scalar load, which does not queue behind the wave's own stores (a vector load would: the
+// compute waves would wait for a tile's stores to drain before starting the next tile).
+// Headers are never written during a launch.
+__device__ __forceinline__ mf::TileHdr load_hdr(const mf::TileHdr *tiles, int i)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    using ConstHdr = const __attribute__((address_space(4))) mf::TileHdr;
+    const ConstHdr *p = (const ConstHdr *)(tiles + i);
+    mf::TileHdr h;
+    h.elem_start = p->elem_start;
+    h.n_elem = p->n_elem;
+    h.node_start = p->node_start;
+    h.n_node = p->n_node;
+    h.n_rows = p->n_rows;
+    h.rinfo_start = p->rinfo_start;
+    h.inc_start = p->inc_start;
+    h.n_inc = p->n_inc;
+    h.task_start = p->task_start;
+    h.n_task = p->n_task;
+    h.n_ent = p->n_ent;
+    h.pad = 0;
+    return h;
+#else
+    return tiles[i];  // the host pass only parses device code
+#endif
+}
+
+// The tiles one XCD walks: each job's tiles cut into 8 contiguous ranges, range x for XCD
+// x, the ranges of XCD x one job after another. Every XCD thus gets an eighth of every job
+// (a launch that mixes cheap lap tiles with costly ela tiles stays balanced), and tiles that
+// share elements stay on one XCD. One job: XCD x walks tiles x*ceil(n/8) ... as it always
+// has.
+template <class Jobs>
+struct XcdWalk {
+    const Jobs &J;
+    int xcd;
+    __device__ int per(int j) const { return (J.at(j).ntiles + kXcd - 1) / kXcd; }
+    __device__ int len(int j) const
+    {
+        const int n = J.at(j).ntiles, p = per(j);
+        return n - xcd * p < 0 ? 0 : (n - xcd * p < p ? n - xcd * p : p);
+    }
+    __device__ int total() const
+    {
+        int L = 0;
+        for (int j = 0; j < J.count(); j++) L += len(j);
+        return L;
+    }
+};
+
+// A position of an XCD's walk: the job it falls in and where that job's range begins.
+struct Cursor {
+    int j, base;
+};
+
+// Moves c to position p (positions never decrease along a walk) and returns the tile's
+// index in job c.j.
+template <class Jobs>
+__device__ __forceinline__ int seek(const XcdWalk<Jobs> &W, Cursor &c, int p)
+{
+    while (p >= c.base + W.len(c.j)) {
+        c.base += W.len(c.j);
+        c.j++;
+    }
+    return W.xcd * W.per(c.j) + p - c.base;
+}
+
+// LDS layout of tile hh as operator ELA runs it. FRONT_ELA: the fixed front part holds
+// ela coefficients (16 bytes per element); a launch that mixes operators sizes it so for
+// every tile, since the loaders fill the next tile's front while the current one runs.
+template <bool ELA, int PREC, bool FRONT_ELA>
+__device__ __forceinline__ mf::TileLds tile_layout(const mf::TileHdr &hh, int f_node, int f_elem)
+{
+    using T = typename std::conditional<PREC == MF_PREC_FP64, double, float>::type;
+    return mf::tile_lds_layout(FRONT_ELA, ELA, f_node, f_elem, hh.n_elem, hh.n_rows, hh.n_inc,
+                               hh.n_task, hh.n_ent,
+                               mf::stage_len(ELA, PREC == MF_PREC_FP64) * (int)sizeof(T));
+}
+
+// One tile on the compute waves (phases A, B, O, W and barriers 1-4), operator ELA; jb
+// holds the tile's output buffers. The loader waves meet the same four barriers.
+template <bool ELA, int PREC, int BT, bool FRONT_ELA>
+__device__ __forceinline__ void compute_tile(const mf::TileHdr &h, const TileJob &jb, int f_node,
+                                             int f_elem, unsigned char *lds_raw)
+{
+    using T = typename std::conditional<PREC == MF_PREC_FP64, double, float>::type;
+    using T2 = typename Vec2<T>::type;
+    constexpr int S = mf::stage_len(ELA, PREC == MF_PREC_FP64);
+    constexpr int NA = ELA ? 9 : 1;
+    constexpr int NC = mf::compute_waves(BT), CT = NC * 64;
+    constexpr int MAXE = mf::max_tile_elems(BT);
+    constexpr int EA = (MAXE + CT - 1) / CT;  // A: elements per thread
+    constexpr int TB = mf::task_blocks_per_wave(ELA);  // task blocks per wave
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int NN = h.n_node, TE = h.n_elem, R = h.n_rows;
+    (void)NN;
+    const mf::TileLds L = tile_layout<ELA, PREC, FRONT_ELA>(h, f_node, f_elem);
+    T *stage = reinterpret_cast<T *>(lds_raw + L.stage);
+    double *outv = reinterpret_cast<double *>(lds_raw + L.stage);  // the output image
+    const int4 *lri = reinterpret_cast<const int4 *>(lds_raw + L.rinfo);
+    const uint4 *ltask = reinterpret_cast<const uint4 *>(lds_raw + L.tasks);
+    const double *lc = reinterpret_cast<const double *>(lds_raw + L.coords);
+    const int *lrnode = reinterpret_cast<const int *>(lds_raw + L.rnode);
+    const uint32_t *lverts = reinterpret_cast<const uint32_t *>(lds_raw + L.verts);
+    const uint16_t *linc = reinterpret_cast<const uint16_t *>(lds_raw + L.inc);
+
+    // ---- A: scaled gradients + coefficients of every tile element, one slot each
+#pragma unroll
+    for (int k = 0; k < EA; k++) {
+        const int t = tid + k * CT;
+        if (t >= TE) continue;
+        const uint32_t vv = lverts[t];
+        const int vi[4] = {(int)(vv & 0xFF), (int)((vv >> 8) & 0xFF), (int)((vv >> 16) & 0xFF),
+                           (int)(vv >> 24)};
+        double c[4][3], g[4][3];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) c[i][d] = lc[vi[i] * 3 + d];
+        mf::elem_grad(c, g);
+        T *sl = stage + t * S;
+        if constexpr (ELA) {
+            // src/assembly.cc:386-409 with per-element (lambda, mu)
+            const double2 lm = reinterpret_cast<const double2 *>(lds_raw + L.coef)[t];
+            const double cd = lm.x + 2.0 * lm.y, co = lm.x + lm.y, cm = lm.y;
+#pragma unroll
+            for (int i = 0; i < 4; i++) ela_vertex_store(sl + 4 * i, g[i], cd);
+            *reinterpret_cast<T2 *>(sl + 16) = T2{(T)co, (T)cm};
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                *reinterpret_cast<T2 *>(sl + 2 * i) = T2{(T)g[i][0], (T)g[i][1]};
+                sl[8 + i] = (T)g[i][2];
+            }
+            const double kk = reinterpret_cast<const double *>(lds_raw + L.coef)[t];
+            *reinterpret_cast<T2 *>(sl + 12) = T2{(T)kk, (T)0};
+        }
+    }
+    // the zero slot (index TE): the target of an unpaired second contribution
+    if (tid < S) stage[TE * S + tid] = (T)0;
+    __syncthreads();  // 1
+
+    // ---- B: one work block per compute wave, results held in registers
+    const int nDB = mf::diag_blocks(ELA, R), nEB = (h.n_task + 63) / 64;
+    const bool isDiag = wave < nDB;
+    const int r = wave * 64 + lane;          // diagonal: the lane's row
+    const bool dOn = isDiag && r < R;
+    int4 dri = make_int4(0, 0, 0, 0);
+    double blk[NA];
+#pragma unroll
+    for (int q = 0; q < NA; q++) blk[q] = 0.0;
+    // task blocks of this wave: (wave - nDB) + k * (NC - nDB), k < TB
+    bool tOn[TB];
+    int tEnt[TB], xEnt[TB];  // tile entries of the task's block and of its transpose (-1: none)
+    double tres[TB][NA];
+#pragma unroll
+    for (int k = 0; k < TB; k++) {
+        const int m = (wave - nDB) + k * (NC - nDB);
+        tOn[k] = !isDiag && m < nEB && m * 64 + lane < h.n_task;
+        tEnt[k] = 0;
+        xEnt[k] = -1;
+#pragma unroll
+        for (int q = 0; q < NA; q++) tres[k][q] = 0.0;
+    }
+    if (isDiag) {
+        // the diagonal chains are the tile's critical path: this wave's instructions
+        // go first on its SIMD while it sums them
+        __builtin_amdgcn_s_setprio(3);
+        dri = dOn ? lri[r] : make_int4(0, 0, 0, 0);
+        const int incOff = dri.z & 0xFFFF, val = dOn ? (dri.z >> 16) & 0xFFFF : 0;
+        // The row's incidence list, two entries per 32-bit word (lists are padded to a
+        // multiple of 4 entries), summed in order one pair at a time. Software
+        // pipelined: the LDS reads of the next pair are in flight while the current
+        // pair is summed; reads past the list repeat its last word and are never
+        // summed.
+        const uint32_t *lw = reinterpret_cast<const uint32_t *>(linc + incOff);
+        const int np = (val + 1) >> 1, lastw = val > 0 ? ((val + 3) >> 2) * 2 - 1 : 0;
+        auto word = [&](int q) { return lw[q < lastw ? q : lastw]; };
+        struct Pair {
+            T g[2][3];
+            T c[2][3];  // ela: lambda + 2 mu, lambda + mu, mu; lap: k
+        };
+        auto fetch = [&](uint32_t w, Pair &P) {
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const int e = u ? (int)(w >> 16) : (int)(w & 0xFFFF);
+                const T *sl = stage + (e >> 2) * S;
+                if constexpr (ELA) {
+                    ela_vertex(sl + 4 * (e & 3), P.g[u], P.c[u][0]);
+                    const T2 c12 = *reinterpret_cast<const T2 *>(sl + 16);
+                    P.c[u][1] = c12.x;
+                    P.c[u][2] = c12.y;
+                } else {
+                    stage_grad<T, T2>(sl, e & 3, P.g[u]);
+                    P.c[u][0] = sl[12];
+                }
+            }
+        };
+        // ela_block(g, g) is exactly symmetric (g_d*g_e == g_e*g_d in IEEE): six
+        // sums hold the block bit for bit
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s4 = 0.0, s5 = 0.0, s8 = 0.0;
+        auto sum = [&](const Pair &P, int p) {
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                if (2 * p + u >= val) break;
+                const T *g = P.g[u];
+                if constexpr (ELA) {
+                    const T cd = P.c[u][0], co = P.c[u][1], cm = P.c[u][2];
+                    const T p00 = g[0] * g[0], p11 = g[1] * g[1], p22 = g[2] * g[2];
+                    s0 += (double)(p00 * cd + p11 * cm + p22 * cm);
+                    s1 += (double)(g[0] * g[1] * co);
+                    s2 += (double)(g[0] * g[2] * co);
+                    s4 += (double)(p00 * cm + p11 * cd + p22 * cm);
+                    s5 += (double)(g[1] * g[2] * co);
+                    s8 += (double)(p00 * cm + p11 * cm + p22 * cd);
+                } else {
+                    const T x = P.c[u][0] * mf::lap_entry(g, g);
+                    s0 += (double)x;
+                }
+            }
+        };
+        // LDS reads complete in issue order: each pair's list word is read before the
+        // previous pair's data, so waiting for the word leaves that data in flight
+        Pair A, B;
+        uint32_t wa = word(0), wb = word(1);
+        fetch(wa, A);  // pair 0
+        for (int p = 0; p < np; p += 2) {
+            wa = word(p + 2);
+            fetch(wb, B);  // pair p + 1
+            sum(A, p);
+            if (p + 1 >= np) break;
+            wb = word(p + 3);
+            fetch(wa, A);  // pair p + 2
+            sum(B, p + 1);
+        }
+        if constexpr (ELA) {
+            blk[0] = s0; blk[1] = s1; blk[2] = s2;
+            blk[3] = s1; blk[4] = s4; blk[5] = s5;
+            blk[6] = s2; blk[7] = s5; blk[8] = s8;
+        } else {
+            blk[0] = s0;
+        }
+        __builtin_amdgcn_s_setprio(0);
+    } else {
+#pragma unroll
+    for (int kb = 0; kb < TB; kb++) {
+        if (!tOn[kb]) continue;
+        // off-diagonal task: the elements holding both a and c, two per iteration
+        // (independent LDS chains, summed in order); the block goes to row a, its
+        // transpose to row c
+        const int ti = ((wave - nDB) + kb * (NC - nDB)) * 64 + lane;
+        double (&acc)[NA] = tres[kb];
+        const uint4 rc = ltask[ti];
+        const int ra = rc.w & 0xFF, oa = (rc.w >> 8) & 0xFF, cl = (rc.w >> 16) & 0xFF;
+        const int4 ri = lri[ra];
+        tEnt[kb] = ri.w + oa;
+        if (cl < R) xEnt[kb] = lri[cl].w + (int)(rc.w >> 24);  // c is a row of this tile
+        // one contribution pair (element slot, vertex slots of a and c); a padding
+        // entry reads the all-zero slot TE, whose products are +0.0 and leave the
+        // sums unchanged
+        auto add_pair = [&](const int (&t)[2], const int (&a)[2], const int (&kk)[2]) {
+            T ga[2][3], gk[2][3], c0[2], c1[2], c2[2];
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const T *sl = stage + t[u] * S;
+                if constexpr (ELA) {
+                    T unused;
+                    ela_vertex(sl + 4 * a[u], ga[u], c0[u]);
+                    ela_vertex(sl + 4 * kk[u], gk[u], unused);
+                    const T2 c12 = *reinterpret_cast<const T2 *>(sl + 16);
+                    c1[u] = c12.x;
+                    c2[u] = c12.y;
+                } else {
+                    stage_grad<T, T2>(sl, a[u], ga[u]);
+                    stage_grad<T, T2>(sl, kk[u], gk[u]);
+                    c0[u] = sl[12];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                if constexpr (ELA) {
+                    T bl[9];
+                    mf::ela_block(ga[u], gk[u], c0[u], c1[u], c2[u], bl);
+#pragma unroll
+                    for (int q = 0; q < 9; q++) acc[q] += (double)bl[q];
+                } else {
+                    const T x = c0[u] * mf::lap_entry(ga[u], gk[u]);
+                    acc[0] += (double)x;
+                }
+            }
+        };
+        if (rc.x == mf::kWalkRecord) {
+            // more than kListLen elements on a row longer than a mask: walk row a's
+            // whole incidence list, taking the elements that hold c, two at a time
+            const int incOff = ri.z & 0xFFFF, val = (ri.z >> 16) & 0xFFFF;
+            int pt = -1, pa = 0, pk = 0;  // a contribution waiting for its pair
+            for (int j = 0; j < val; j++) {
+                const int e = linc[incOff + j], te = e >> 2;
+                const uint32_t vv = lverts[te];
+                const int kc = (int)(vv & 0xFF) == cl ? 0
+                               : (int)((vv >> 8) & 0xFF) == cl ? 1
+                               : (int)((vv >> 16) & 0xFF) == cl ? 2
+                               : (int)(vv >> 24) == cl ? 3 : -1;
+                if (kc < 0) continue;
+                if (pt < 0) {
+                    pt = te;
+                    pa = e & 3;
+                    pk = kc;
+                    continue;
+                }
+                const int t[2] = {pt, te}, a[2] = {pa, e & 3}, kk[2] = {pk, kc};
+                add_pair(t, a, kk);
+                pt = -1;
+            }
+            if (pt >= 0) {  // an odd count: the last one with the zero slot
+                const int t[2] = {pt, TE}, a[2] = {pa, 0}, kk[2] = {pk, 0};
+                add_pair(t, a, kk);
+            }
+        } else if (rc.x != mf::kMaskRecord) {
+            // up to kListLen contributions, padded with the zero slot TE
+            const uint32_t w[3] = {rc.x, rc.y, rc.z};
+#pragma unroll
+            for (int u = 0; u < 3; u++) {
+                const int e0 = (int)(w[u] & 0xFFFF), e1 = (int)(w[u] >> 16);
+                if ((e0 >> 4) == TE) break;
+                const int t[2] = {e0 >> 4, e1 >> 4}, a[2] = {(e0 >> 2) & 3, (e1 >> 2) & 3},
+                          kk[2] = {e0 & 3, e1 & 3};
+                add_pair(t, a, kk);
+            }
+        } else {
+            // more than kListLen elements: walk the mask over row a's incidence list
+            uint64_t msk = (uint64_t)rc.y | (uint64_t)rc.z << 32;
+            const int incOff = ri.z & 0xFFFF;
+            while (msk) {
+                const int b0 = __builtin_ctzll(msk);
+                msk &= msk - 1;
+                const bool two = msk != 0;
+                const int b1 = two ? __builtin_ctzll(msk) : b0;
+                if (two) msk &= msk - 1;
+                const int e0 = linc[incOff + b0], e1 = linc[incOff + b1];
+                const int t[2] = {e0 >> 2, two ? e1 >> 2 : TE}, a[2] = {e0 & 3, e1 & 3};
+                int kk[2];
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const uint32_t vv = lverts[t[u]];
+                    kk[u] = (int)(vv & 0xFF) == cl ? 0
+                            : (int)((vv >> 8) & 0xFF) == cl ? 1
+                            : (int)((vv >> 16) & 0xFF) == cl ? 2 : 3;
+                }
+                add_pair(t, a, kk);
+            }
+        }
+    }
+    }
+    __syncthreads();  // 2: every stage read is done
+
+    // ---- O: the blocks (and in-tile transposes) into the output image
+    if (dOn) {
+        double *o = outv + (int64_t)(dri.w + ((dri.y >> 16) & 0xFF)) * NA;
+#pragma unroll
+        for (int q = 0; q < NA; q++) o[q] = blk[q];
+    }
+#pragma unroll
+    for (int kb = 0; kb < TB; kb++) {
+        if (!tOn[kb]) continue;
+        const double (&r)[NA] = tres[kb];
+        double *o = outv + (int64_t)tEnt[kb] * NA;
+#pragma unroll
+        for (int q = 0; q < NA; q++) o[q] = r[q];
+        if (xEnt[kb] >= 0) {
+            if constexpr (ELA) {
+                const double tr[9] = {r[0], r[3], r[6], r[1], r[4],
+                                      r[7], r[2], r[5], r[8]};  // transpose
+                double *ot = outv + (int64_t)xEnt[kb] * 9;
+#pragma unroll
+                for (int q = 0; q < 9; q++) ot[q] = tr[q];
+            } else {
+                outv[xEnt[kb]] = r[0];
+            }
+        }
+    }
+    __syncthreads();  // 3
+
+    // ---- W: the diagonal waves invert their blocks into prec; the other compute
+    // waves copy the image to the rows' CSR segments (each segment contiguous in both)
+    if (isDiag) {
+        if (dOn) {
+            const int node = lrnode[r];
+            if constexpr (ELA) {
+                const int info = mf::ela_invert_block(blk, (dri.y >> 24) & 7, true);
+                store_block9(jb.prec + (int64_t)node * 9, blk);
+                if (info != 0 && jb.singular) atomicAdd(jb.singular, 1);
+            } else {
+                jb.prec[node] = 1.0 / blk[0];
+            }
+        }
+    } else {
+        const int nwr = NC - nDB, wr = wave - nDB;
+        if constexpr (ELA) {
+            // four rows per wave at a time, 16 lanes each. A row is 9 len doubles
+            // (~135) moved as 16-byte pieces, one 8-byte piece at the end of an
+            // odd-length row (8-byte aligned 16-byte stores are valid on gfx950; the
+            // LDS side reads two 8-byte words). A lane reads all its pieces of a pass
+            // (CP of them, 32 CP doubles per row and pass) before it stores any, so
+            // a row costs one LDS round trip, not one per piece.
+            constexpr int CP = 5;
+            const int g = lane >> 4, l16 = lane & 15;
+            for (int rr = wr * 4 + g; rr < R; rr += nwr * 4) {
+                const int4 ri = lri[rr];
+                const int n = (ri.y & 0xFFFF) * 9;
+                const double *src = outv + (int64_t)ri.w * 9;
+                double *dst = jb.values + (int64_t)ri.x * 9;
+                for (int j0 = 2 * l16; j0 < n; j0 += 32 * CP) {
+                    double2 v[CP];
+#pragma unroll
+                    for (int k = 0; k < CP; k++) {
+                        const int j = j0 + 32 * k;
+                        v[k] = make_double2(j < n ? src[j] : 0.0, j + 1 < n ? src[j + 1] : 0.0);
+                    }
+#pragma unroll
+                    for (int k = 0; k < CP; k++) {
+                        const int j = j0 + 32 * k;
+                        if (j + 1 < n)
+                            *reinterpret_cast<double2 *>(dst + j) = v[k];
+                        else if (j < n)
+                            dst[j] = v[k].x;
+                    }
+                }
+            }
+        } else {
+            // four rows per wave, 16 lanes each
+            const int g = lane >> 4, l16 = lane & 15;
+            for (int rr = wr * 4 + g; rr < R; rr += nwr * 4) {
+                const int4 ri = lri[rr];
+                const int n = ri.y & 0xFFFF;
+                for (int j = l16; j < n; j += 16) jb.values[(int64_t)ri.x + j] = outv[ri.w + j];
+            }
+        }
+    }
+    // the next tile's vertices, coordinates and coefficients are written meanwhile
+    __syncthreads();  // 4
+}
+
+// Persistent workgroups, XCD-aware. The tiles of a launch (one plan's, or the plans of a
+// job table one after another) are cut into 8 contiguous ranges in that order, range x
+// for XCD x (workgroup b runs on XCD b % 8); the G workgroups of an XCD walk its range
+// together, slot s0, s0 + G, ..., so tiles that share elements and nodes run on one XCD
+// close in time and share its L2.
 //
 // Waves 0..NC-1 compute, waves NC.. load. Per tile, separated by four barriers:
 //   A  compute: element stage          | load: row data, tasks, incidences into LDS
@@ -110,42 +598,41 @@ __device__ __forceinline__ void ela_vertex_store(float *p, const double (&g)[3],
 //   W  compute: image -> values, prec  | load: wait gathers, next vertices/coords/coefs
 // Compute waves issue no global loads and loader waves no stores, so no wave ever waits
 // for its own stores to drain.
-template <int OP, int PREC, int BT>
-__global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
-    k_tiled(const mf::TileHdr *__restrict__ tiles, int ntiles, int G, int f_node, int f_elem,
-            const int32_t *__restrict__ tile_elems, const uint32_t *__restrict__ tile_verts,
-            const int32_t *__restrict__ tile_nodes, const int4 *__restrict__ rowinfo,
-            const uint4 *__restrict__ inc8, const uint4 *__restrict__ task_rec,
-            const double *__restrict__ coord,
-            const double *__restrict__ coef, double *__restrict__ values,
-            double *__restrict__ prec, int *__restrict__ singular)
+template <int OPS, int PREC, int BT, class Jobs>
+__device__ __forceinline__ void tiled_body(const Jobs &J, int G, int f_node, int f_elem)
 {
-    using T = typename std::conditional<PREC == MF_PREC_FP64, double, float>::type;
-    using T2 = typename Vec2<T>::type;
-    constexpr bool ELA = OP == MF_OP_ELA;
-    constexpr int S = mf::stage_len(ELA, PREC == MF_PREC_FP64);
-    constexpr int NA = ELA ? 9 : 1;
+    constexpr bool FRONT_ELA = OPS != MF_OP_LAP;
     constexpr int NC = mf::compute_waves(BT), CT = NC * 64, LT = BT - CT;
     constexpr int MAXE = mf::max_tile_elems(BT);
-    constexpr int EA = (MAXE + CT - 1) / CT;                              // A: elements per thread
     constexpr int NPT = (mf::kMaxTileNodes + LT - 1) / LT;               // loader: nodes, rows
     constexpr int EPT = (MAXE + LT - 1) / LT;                            // loader: elements
     constexpr int IPT = (mf::kMaxTileInc / 8 + LT - 1) / LT;             // loader: 16-byte incidences
-    constexpr int TPT = (mf::max_tile_tasks(ELA, BT) + LT - 1) / LT;     // loader: tasks
-    constexpr int TB = mf::task_blocks_per_wave(ELA);                   // task blocks per wave
+    // loader: tasks (a launch that mixes operators holds the larger lap count)
+    constexpr int TPT = (mf::max_tile_tasks(OPS == MF_OP_ELA, BT) + LT - 1) / LT;
     static_assert(mf::kMaxTileRows <= mf::kMaxTileNodes, "rows are the first tile nodes");
     extern __shared__ __align__(16) unsigned char lds_raw[];
 
-    const int per = (ntiles + kXcd - 1) / kXcd;
     const int xcd = blockIdx.x % kXcd;
+    const XcdWalk<Jobs> W{J, xcd};
+    const int nslots = W.total();  // this XCD's tiles
     int slot = blockIdx.x / kXcd;
-    if (slot >= per || xcd * per + slot >= ntiles) return;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    auto layout = [&](const mf::TileHdr &hh) {
-        return mf::tile_lds_layout(ELA, f_node, f_elem, hh.n_elem, hh.n_rows, hh.n_inc,
-                                   hh.n_task, hh.n_ent, S * (int)sizeof(T));
+    if (slot >= nslots) return;
+    const int tid = threadIdx.x, wave = tid >> 6;
+    auto is_ela = [&](const TileJob &jb) -> bool {
+        if constexpr (OPS == kOpsAny) return jb.op == MF_OP_ELA;
+        else return OPS == MF_OP_ELA;
     };
-    mf::TileHdr h = tiles[xcd * per + slot];
+    auto layout = [&](const mf::TileHdr &hh, bool ela) {
+        return ela ? tile_layout<true, PREC, FRONT_ELA>(hh, f_node, f_elem)
+                   : tile_layout<false, PREC, FRONT_ELA>(hh, f_node, f_elem);
+    };
+    Cursor cur{0, 0};
+    mf::TileHdr h = load_hdr(J.at(0).tiles, 0);
+    {
+        const int t = seek(W, cur, slot);
+        h = load_hdr(J.at(cur.j).tiles, t);
+    }
+    int jcur = cur.j;
 
     if (wave >= NC) {
         // ------------------------------------------------------------------ loader waves
@@ -168,17 +655,20 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
         auto at = [](int start, int n, int i) -> int64_t {
             return n > 0 ? (int64_t)start + (i < n ? i : n - 1) : 0;
         };
-        auto load_ids = [&](const mf::TileHdr &hh, int (&nid)[NPT], int (&eid)[EPT]) {
+        auto load_ids = [&](const TileJob &jb, const mf::TileHdr &hh, int (&nid)[NPT],
+                            int (&eid)[EPT]) {
 #pragma unroll
-            for (int r = 0; r < NPT; r++) nid[r] = tile_nodes[at(hh.node_start, hh.n_node, lt + r * LT)];
+            for (int r = 0; r < NPT; r++)
+                nid[r] = jb.tile_nodes[at(hh.node_start, hh.n_node, lt + r * LT)];
 #pragma unroll
-            for (int r = 0; r < EPT; r++) eid[r] = tile_elems[at(hh.elem_start, hh.n_elem, lt + r * LT)];
+            for (int r = 0; r < EPT; r++)
+                eid[r] = jb.tile_elems[at(hh.elem_start, hh.n_elem, lt + r * LT)];
         };
-        auto load_static = [&](const mf::TileHdr &hh) {
+        auto load_static = [&](const TileJob &jb, const mf::TileHdr &hh) {
 #pragma unroll
             for (int r = 0; r < NPT; r++) {
                 const int i = lt + r * LT;
-                const int4 v = rowinfo[at(hh.rinfo_start, hh.n_rows, i)];
+                const int4 v = jb.rowinfo[at(hh.rinfo_start, hh.n_rows, i)];
                 rix[r] = v.x;
                 riy[r] = v.y;
                 riz[r] = v.z;
@@ -187,11 +677,11 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
             for (int r = 0; r < EPT; r++) {
                 const int i = lt + r * LT;
-                vert[r] = tile_verts[at(hh.elem_start, hh.n_elem, i)];
+                vert[r] = jb.tile_verts[at(hh.elem_start, hh.n_elem, i)];
             }
 #pragma unroll
             for (int r = 0; r < IPT; r++) {
-                const uint4 v = inc8[at(hh.inc_start / 8, hh.n_inc / 8, lt + r * LT)];
+                const uint4 v = jb.inc8[at(hh.inc_start / 8, hh.n_inc / 8, lt + r * LT)];
                 inx[r] = v.x;
                 iny[r] = v.y;
                 inz[r] = v.z;
@@ -200,14 +690,15 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
             for (int r = 0; r < TPT; r++) {
                 const int64_t e = at(hh.task_start, hh.n_task, lt + r * LT);
-                const uint4 v = task_rec[e];
+                const uint4 v = jb.task_rec[e];
                 rcx[r] = v.x;
                 rcy[r] = v.y;
                 rcz[r] = v.z;
                 rcw[r] = v.w;
             }
         };
-        auto load_gathers = [&]() {
+        auto load_gathers = [&](const TileJob &jb) {
+            const double *coord = jb.coord, *coef = jb.coef;
 #pragma unroll
             for (int r = 0; r < NPT; r++) {
                 const double *p = coord + (int64_t)nodeId[r] * 3;
@@ -216,31 +707,46 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                 cx[r][2] = p[2];
             }
             // without a coefficient array, a valid dummy address keeps the loads
-            // branch-free (fill1 substitutes the constants)
+            // branch-free (fill1 substitutes the constants). The operator branch stays
+            // outside the unrolled loops, so k0/k1 are indexed by constants only.
             const bool hc = coef != nullptr;
+            if constexpr (PREC == MF_PREC_MIXED_F32COEF) {  // a float coefficient field
+                const float *cf = reinterpret_cast<const float *>(hc ? coef : coord);
+                if (is_ela(jb)) {
 #pragma unroll
-            for (int r = 0; r < EPT; r++) {
-                if constexpr (PREC == MF_PREC_MIXED_F32COEF) {  // a float coefficient field
-                    const float *cf = reinterpret_cast<const float *>(hc ? coef : coord);
-                    if constexpr (ELA) {
+                    for (int r = 0; r < EPT; r++) {
                         const float2 lm = reinterpret_cast<const float2 *>(cf)[hc ? elemId[r] : 0];
                         k0[r] = lm.x;
                         k1[r] = lm.y;
-                    } else {
-                        k0[r] = cf[hc ? elemId[r] : 0];
                     }
-                } else if constexpr (ELA) {
-                    const double2 lm = reinterpret_cast<const double2 *>(hc ? coef : coord)[hc ? elemId[r] : 0];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < EPT; r++) {
+                        k0[r] = cf[hc ? elemId[r] : 0];
+                        k1[r] = 0.0;  // both branches define every register of k0 and k1
+                    }
+                }
+            } else if (is_ela(jb)) {
+                const double2 *cf = reinterpret_cast<const double2 *>(hc ? coef : coord);
+#pragma unroll
+                for (int r = 0; r < EPT; r++) {
+                    const double2 lm = cf[hc ? elemId[r] : 0];
                     k0[r] = lm.x;
                     k1[r] = lm.y;
-                } else {
-                    k0[r] = (hc ? coef : coord)[hc ? elemId[r] : 0];
+                }
+            } else {
+                const double *cf = hc ? coef : coord;
+#pragma unroll
+                for (int r = 0; r < EPT; r++) {
+                    k0[r] = cf[hc ? elemId[r] : 0];
+                    k1[r] = 0.0;
                 }
             }
         };
         // vertices, coordinates, coefficients: what A reads
-        auto fill1 = [&](const mf::TileHdr &hh) {
-            const mf::TileLds L = layout(hh);
+        auto fill1 = [&](const TileJob &jb, const mf::TileHdr &hh) {
+            const bool ela = is_ela(jb);
+            const mf::TileLds L = layout(hh, ela);
             double *lc = reinterpret_cast<double *>(lds_raw + L.coords);
             uint32_t *lverts = reinterpret_cast<uint32_t *>(lds_raw + L.verts);
 #pragma unroll
@@ -253,24 +759,32 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
                 }
             }
             // without a coefficient array: the reference constants (lambda 0.25, mu 1 | k 1)
-            const bool hc = coef != nullptr;
+            const bool hc = jb.coef != nullptr;
 #pragma unroll
             for (int r = 0; r < EPT; r++) {
                 const int i = lt + r * LT;
-                if (i < hh.n_elem) {
-                    lverts[i] = vert[r];
-                    if constexpr (ELA)
-                        reinterpret_cast<double2 *>(lds_raw + L.coef)[i] =
-                            make_double2(hc ? k0[r] : 0.25, hc ? k1[r] : 1.0);
-                    else
-                        reinterpret_cast<double *>(lds_raw + L.coef)[i] = hc ? k0[r] : 1.0;
+                if (i < hh.n_elem) lverts[i] = vert[r];
+            }
+            if (ela) {
+                double2 *lk = reinterpret_cast<double2 *>(lds_raw + L.coef);
+#pragma unroll
+                for (int r = 0; r < EPT; r++) {
+                    const int i = lt + r * LT;
+                    if (i < hh.n_elem) lk[i] = make_double2(hc ? k0[r] : 0.25, hc ? k1[r] : 1.0);
+                }
+            } else {
+                double *lk = reinterpret_cast<double *>(lds_raw + L.coef);
+#pragma unroll
+                for (int r = 0; r < EPT; r++) {
+                    const int i = lt + r * LT;
+                    if (i < hh.n_elem) lk[i] = hc ? k0[r] : 1.0;
                 }
             }
             if (lt == 0) lverts[hh.n_elem] = 0u;  // the zero slot's vertices
         };
         // row info, row nodes, incidence lists, tasks: what B, O and W read
-        auto fill2 = [&](const mf::TileHdr &hh) {
-            const mf::TileLds L = layout(hh);
+        auto fill2 = [&](const TileJob &jb, const mf::TileHdr &hh) {
+            const mf::TileLds L = layout(hh, is_ela(jb));
             int4 *lri = reinterpret_cast<int4 *>(lds_raw + L.rinfo);
             int *lrnode = reinterpret_cast<int *>(lds_raw + L.rnode);
 #pragma unroll
@@ -296,16 +810,23 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
             }
         };
 
-        auto valid = [&](int sl) { return sl < per && xcd * per + sl < ntiles; };
-        load_ids(h, nodeId, elemId);
-        load_static(h);
-        load_gathers();
-        if (valid(slot + G)) load_ids(tiles[xcd * per + slot + G], nodeIdN, elemIdN);
-        fill1(h);
+        auto valid = [&](int sl) { return sl < nslots; };
+        // jcur: the job of tile h; cIds, tIds: the walk position and tile index of the tile
+        // whose ids are in nodeIdN
+        Cursor cIds = cur;
+        int tIds = 0;
+        load_ids(J.at(jcur), h, nodeId, elemId);
+        load_static(J.at(jcur), h);
+        load_gathers(J.at(jcur));
+        if (valid(slot + G)) {
+            tIds = seek(W, cIds, slot + G);
+            load_ids(J.at(cIds.j), load_hdr(J.at(cIds.j).tiles, tIds), nodeIdN, elemIdN);
+        }
+        fill1(J.at(jcur), h);
         __syncthreads();  // P
         for (;;) {
             // A
-            fill2(h);
+            fill2(J.at(jcur), h);
             const int nslot = slot + G;
             const bool has_next = valid(nslot);
             if (has_next) {  // the next tile's ids, loaded a tile ago
@@ -317,20 +838,26 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
             __syncthreads();  // 1
             // B: the next tile's static data and gathers, and the ids of the one after
             mf::TileHdr hn = h;
+            int jn = jcur;
             if (has_next) {
-                hn = tiles[xcd * per + nslot];
-                load_static(hn);
-                load_gathers();
-                if (valid(nslot + G)) load_ids(tiles[xcd * per + nslot + G], nodeIdN, elemIdN);
+                jn = cIds.j;
+                hn = load_hdr(J.at(jn).tiles, tIds);
+                load_static(J.at(jn), hn);
+                load_gathers(J.at(jn));
+                if (valid(nslot + G)) {
+                    tIds = seek(W, cIds, nslot + G);
+                    load_ids(J.at(cIds.j), load_hdr(J.at(cIds.j).tiles, tIds), nodeIdN, elemIdN);
+                }
             }
             __syncthreads();  // 2
             // O
             __syncthreads();  // 3
             // W
-            if (has_next) fill1(hn);
+            if (has_next) fill1(J.at(jn), hn);
             __syncthreads();  // 4
             if (!has_next) break;
             h = hn;
+            jcur = jn;
             slot = nslot;
         }
         return;
@@ -339,358 +866,31 @@ __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
     // ---------------------------------------------------------------------- compute waves
     __syncthreads();  // P
     for (;;) {
-        const int NN = h.n_node, TE = h.n_elem, R = h.n_rows;
-        (void)NN;
-        const mf::TileLds L = layout(h);
-        T *stage = reinterpret_cast<T *>(lds_raw + L.stage);
-        double *outv = reinterpret_cast<double *>(lds_raw + L.stage);  // the output image
-        const int4 *lri = reinterpret_cast<const int4 *>(lds_raw + L.rinfo);
-        const uint4 *ltask = reinterpret_cast<const uint4 *>(lds_raw + L.tasks);
-        const double *lc = reinterpret_cast<const double *>(lds_raw + L.coords);
-        const int *lrnode = reinterpret_cast<const int *>(lds_raw + L.rnode);
-        const uint32_t *lverts = reinterpret_cast<const uint32_t *>(lds_raw + L.verts);
-        const uint16_t *linc = reinterpret_cast<const uint16_t *>(lds_raw + L.inc);
-
-        // ---- A: scaled gradients + coefficients of every tile element, one slot each
-#pragma unroll
-        for (int k = 0; k < EA; k++) {
-            const int t = tid + k * CT;
-            if (t >= TE) continue;
-            const uint32_t vv = lverts[t];
-            const int vi[4] = {(int)(vv & 0xFF), (int)((vv >> 8) & 0xFF), (int)((vv >> 16) & 0xFF),
-                               (int)(vv >> 24)};
-            double c[4][3], g[4][3];
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int d = 0; d < 3; d++) c[i][d] = lc[vi[i] * 3 + d];
-            mf::elem_grad(c, g);
-            T *sl = stage + t * S;
-            if constexpr (ELA) {
-                // src/assembly.cc:386-409 with per-element (lambda, mu)
-                const double2 lm = reinterpret_cast<const double2 *>(lds_raw + L.coef)[t];
-                const double cd = lm.x + 2.0 * lm.y, co = lm.x + lm.y, cm = lm.y;
-#pragma unroll
-                for (int i = 0; i < 4; i++) ela_vertex_store(sl + 4 * i, g[i], cd);
-                *reinterpret_cast<T2 *>(sl + 16) = T2{(T)co, (T)cm};
-            } else {
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    *reinterpret_cast<T2 *>(sl + 2 * i) = T2{(T)g[i][0], (T)g[i][1]};
-                    sl[8 + i] = (T)g[i][2];
-                }
-                const double kk = reinterpret_cast<const double *>(lds_raw + L.coef)[t];
-                *reinterpret_cast<T2 *>(sl + 12) = T2{(T)kk, (T)0};
-            }
-        }
-        // the zero slot (index TE): the target of an unpaired second contribution
-        if (tid < S) stage[TE * S + tid] = (T)0;
-        __syncthreads();  // 1
-
-        // ---- B: one work block per compute wave, results held in registers
-        const int nDB = mf::diag_blocks(ELA, R), nEB = (h.n_task + 63) / 64;
-        const bool isDiag = wave < nDB;
-        const int r = wave * 64 + lane;          // diagonal: the lane's row
-        const bool dOn = isDiag && r < R;
-        int4 dri = make_int4(0, 0, 0, 0);
-        double blk[NA];
-#pragma unroll
-        for (int q = 0; q < NA; q++) blk[q] = 0.0;
-        // task blocks of this wave: (wave - nDB) + k * (NC - nDB), k < TB
-        bool tOn[TB];
-        int tEnt[TB], xEnt[TB];  // tile entries of the task's block and of its transpose (-1: none)
-        double tres[TB][NA];
-#pragma unroll
-        for (int k = 0; k < TB; k++) {
-            const int m = (wave - nDB) + k * (NC - nDB);
-            tOn[k] = !isDiag && m < nEB && m * 64 + lane < h.n_task;
-            tEnt[k] = 0;
-            xEnt[k] = -1;
-#pragma unroll
-            for (int q = 0; q < NA; q++) tres[k][q] = 0.0;
-        }
-        if (isDiag) {
-            // the diagonal chains are the tile's critical path: this wave's instructions
-            // go first on its SIMD while it sums them
-            __builtin_amdgcn_s_setprio(3);
-            dri = dOn ? lri[r] : make_int4(0, 0, 0, 0);
-            const int incOff = dri.z & 0xFFFF, val = dOn ? (dri.z >> 16) & 0xFFFF : 0;
-            // The row's incidence list, two entries per 32-bit word (lists are padded to a
-            // multiple of 4 entries), summed in order one pair at a time. Software
-            // pipelined: the LDS reads of the next pair are in flight while the current
-            // pair is summed; reads past the list repeat its last word and are never
-            // summed.
-            const uint32_t *lw = reinterpret_cast<const uint32_t *>(linc + incOff);
-            const int np = (val + 1) >> 1, lastw = val > 0 ? ((val + 3) >> 2) * 2 - 1 : 0;
-            auto word = [&](int q) { return lw[q < lastw ? q : lastw]; };
-            struct Pair {
-                T g[2][3];
-                T c[2][3];  // ela: lambda + 2 mu, lambda + mu, mu; lap: k
-            };
-            auto fetch = [&](uint32_t w, Pair &P) {
-#pragma unroll
-                for (int u = 0; u < 2; u++) {
-                    const int e = u ? (int)(w >> 16) : (int)(w & 0xFFFF);
-                    const T *sl = stage + (e >> 2) * S;
-                    if constexpr (ELA) {
-                        ela_vertex(sl + 4 * (e & 3), P.g[u], P.c[u][0]);
-                        const T2 c12 = *reinterpret_cast<const T2 *>(sl + 16);
-                        P.c[u][1] = c12.x;
-                        P.c[u][2] = c12.y;
-                    } else {
-                        stage_grad<T, T2>(sl, e & 3, P.g[u]);
-                        P.c[u][0] = sl[12];
-                    }
-                }
-            };
-            // ela_block(g, g) is exactly symmetric (g_d*g_e == g_e*g_d in IEEE): six
-            // sums hold the block bit for bit
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s4 = 0.0, s5 = 0.0, s8 = 0.0;
-            auto sum = [&](const Pair &P, int p) {
-#pragma unroll
-                for (int u = 0; u < 2; u++) {
-                    if (2 * p + u >= val) break;
-                    const T *g = P.g[u];
-                    if constexpr (ELA) {
-                        const T cd = P.c[u][0], co = P.c[u][1], cm = P.c[u][2];
-                        const T p00 = g[0] * g[0], p11 = g[1] * g[1], p22 = g[2] * g[2];
-                        s0 += (double)(p00 * cd + p11 * cm + p22 * cm);
-                        s1 += (double)(g[0] * g[1] * co);
-                        s2 += (double)(g[0] * g[2] * co);
-                        s4 += (double)(p00 * cm + p11 * cd + p22 * cm);
-                        s5 += (double)(g[1] * g[2] * co);
-                        s8 += (double)(p00 * cm + p11 * cm + p22 * cd);
-                    } else {
-                        const T x = P.c[u][0] * mf::lap_entry(g, g);
-                        s0 += (double)x;
-                    }
-                }
-            };
-            // LDS reads complete in issue order: each pair's list word is read before the
-            // previous pair's data, so waiting for the word leaves that data in flight
-            Pair A, B;
-            uint32_t wa = word(0), wb = word(1);
-            fetch(wa, A);  // pair 0
-            for (int p = 0; p < np; p += 2) {
-                wa = word(p + 2);
-                fetch(wb, B);  // pair p + 1
-                sum(A, p);
-                if (p + 1 >= np) break;
-                wb = word(p + 3);
-                fetch(wa, A);  // pair p + 2
-                sum(B, p + 1);
-            }
-            if constexpr (ELA) {
-                blk[0] = s0; blk[1] = s1; blk[2] = s2;
-                blk[3] = s1; blk[4] = s4; blk[5] = s5;
-                blk[6] = s2; blk[7] = s5; blk[8] = s8;
-            } else {
-                blk[0] = s0;
-            }
-            __builtin_amdgcn_s_setprio(0);
-        } else {
-#pragma unroll
-        for (int kb = 0; kb < TB; kb++) {
-            if (!tOn[kb]) continue;
-            // off-diagonal task: the elements holding both a and c, two per iteration
-            // (independent LDS chains, summed in order); the block goes to row a, its
-            // transpose to row c
-            const int ti = ((wave - nDB) + kb * (NC - nDB)) * 64 + lane;
-            double (&acc)[NA] = tres[kb];
-            const uint4 rc = ltask[ti];
-            const int ra = rc.w & 0xFF, oa = (rc.w >> 8) & 0xFF, cl = (rc.w >> 16) & 0xFF;
-            const int4 ri = lri[ra];
-            tEnt[kb] = ri.w + oa;
-            if (cl < R) xEnt[kb] = lri[cl].w + (int)(rc.w >> 24);  // c is a row of this tile
-            // one contribution pair (element slot, vertex slots of a and c); a padding
-            // entry reads the all-zero slot TE, whose products are +0.0 and leave the
-            // sums unchanged
-            auto add_pair = [&](const int (&t)[2], const int (&a)[2], const int (&kk)[2]) {
-                T ga[2][3], gk[2][3], c0[2], c1[2], c2[2];
-#pragma unroll
-                for (int u = 0; u < 2; u++) {
-                    const T *sl = stage + t[u] * S;
-                    if constexpr (ELA) {
-                        T unused;
-                        ela_vertex(sl + 4 * a[u], ga[u], c0[u]);
-                        ela_vertex(sl + 4 * kk[u], gk[u], unused);
-                        const T2 c12 = *reinterpret_cast<const T2 *>(sl + 16);
-                        c1[u] = c12.x;
-                        c2[u] = c12.y;
-                    } else {
-                        stage_grad<T, T2>(sl, a[u], ga[u]);
-                        stage_grad<T, T2>(sl, kk[u], gk[u]);
-                        c0[u] = sl[12];
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < 2; u++) {
-                    if constexpr (ELA) {
-                        T bl[9];
-                        mf::ela_block(ga[u], gk[u], c0[u], c1[u], c2[u], bl);
-#pragma unroll
-                        for (int q = 0; q < 9; q++) acc[q] += (double)bl[q];
-                    } else {
-                        const T x = c0[u] * mf::lap_entry(ga[u], gk[u]);
-                        acc[0] += (double)x;
-                    }
-                }
-            };
-            if (rc.x != mf::kMaskRecord) {
-                // up to kListLen contributions, padded with the zero slot TE
-                const uint32_t w[3] = {rc.x, rc.y, rc.z};
-#pragma unroll
-                for (int u = 0; u < 3; u++) {
-                    const int e0 = (int)(w[u] & 0xFFFF), e1 = (int)(w[u] >> 16);
-                    if ((e0 >> 4) == TE) break;
-                    const int t[2] = {e0 >> 4, e1 >> 4}, a[2] = {(e0 >> 2) & 3, (e1 >> 2) & 3},
-                              kk[2] = {e0 & 3, e1 & 3};
-                    add_pair(t, a, kk);
-                }
-            } else if (rc.x == mf::kWalkRecord) {
-                // more than kListLen elements on a row longer than a mask: walk row a's
-                // whole incidence list, taking the elements that hold c, two at a time
-                const int incOff = ri.z & 0xFFFF, val = (ri.z >> 16) & 0xFFFF;
-                int pt = -1, pa = 0, pk = 0;  // a contribution waiting for its pair
-                for (int j = 0; j < val; j++) {
-                    const int e = linc[incOff + j], te = e >> 2;
-                    const uint32_t vv = lverts[te];
-                    const int kc = (int)(vv & 0xFF) == cl ? 0
-                                   : (int)((vv >> 8) & 0xFF) == cl ? 1
-                                   : (int)((vv >> 16) & 0xFF) == cl ? 2
-                                   : (int)(vv >> 24) == cl ? 3 : -1;
-                    if (kc < 0) continue;
-                    if (pt < 0) {
-                        pt = te;
-                        pa = e & 3;
-                        pk = kc;
-                        continue;
-                    }
-                    const int t[2] = {pt, te}, a[2] = {pa, e & 3}, kk[2] = {pk, kc};
-                    add_pair(t, a, kk);
-                    pt = -1;
-                }
-                if (pt >= 0) {  // an odd count: the last one with the zero slot
-                    const int t[2] = {pt, TE}, a[2] = {pa, 0}, kk[2] = {pk, 0};
-                    add_pair(t, a, kk);
-                }
-            } else {
-                // more than kListLen elements: walk the mask over row a's incidence list
-                uint64_t msk = (uint64_t)rc.y | (uint64_t)rc.z << 32;
-                const int incOff = ri.z & 0xFFFF;
-                while (msk) {
-                    const int b0 = __builtin_ctzll(msk);
-                    msk &= msk - 1;
-                    const bool two = msk != 0;
-                    const int b1 = two ? __builtin_ctzll(msk) : b0;
-                    if (two) msk &= msk - 1;
-                    const int e0 = linc[incOff + b0], e1 = linc[incOff + b1];
-                    const int t[2] = {e0 >> 2, two ? e1 >> 2 : TE}, a[2] = {e0 & 3, e1 & 3};
-                    int kk[2];
-#pragma unroll
-                    for (int u = 0; u < 2; u++) {
-                        const uint32_t vv = lverts[t[u]];
-                        kk[u] = (int)(vv & 0xFF) == cl ? 0
-                                : (int)((vv >> 8) & 0xFF) == cl ? 1
-                                : (int)((vv >> 16) & 0xFF) == cl ? 2 : 3;
-                    }
-                    add_pair(t, a, kk);
-                }
-            }
-        }
-        }
-        __syncthreads();  // 2: every stage read is done
-
-        // ---- O: the blocks (and in-tile transposes) into the output image
-        if (dOn) {
-            double *o = outv + (int64_t)(dri.w + ((dri.y >> 16) & 0xFF)) * NA;
-#pragma unroll
-            for (int q = 0; q < NA; q++) o[q] = blk[q];
-        }
-#pragma unroll
-        for (int kb = 0; kb < TB; kb++) {
-            if (!tOn[kb]) continue;
-            const double (&r)[NA] = tres[kb];
-            double *o = outv + (int64_t)tEnt[kb] * NA;
-#pragma unroll
-            for (int q = 0; q < NA; q++) o[q] = r[q];
-            if (xEnt[kb] >= 0) {
-                if constexpr (ELA) {
-                    const double tr[9] = {r[0], r[3], r[6], r[1], r[4],
-                                          r[7], r[2], r[5], r[8]};  // transpose
-                    double *ot = outv + (int64_t)xEnt[kb] * 9;
-#pragma unroll
-                    for (int q = 0; q < 9; q++) ot[q] = tr[q];
-                } else {
-                    outv[xEnt[kb]] = r[0];
-                }
-            }
-        }
-        __syncthreads();  // 3
-
-        // ---- W: the diagonal waves invert their blocks into prec; the other compute
-        // waves copy the image to the rows' CSR segments (each segment contiguous in both)
-        if (isDiag) {
-            if (dOn) {
-                const int node = lrnode[r];
-                if constexpr (ELA) {
-                    const int info = mf::ela_invert_block(blk, (dri.y >> 24) & 7, true);
-                    store_block9(prec + (int64_t)node * 9, blk);
-                    if (info != 0 && singular) atomicAdd(singular, 1);
-                } else {
-                    prec[node] = 1.0 / blk[0];
-                }
-            }
-        } else {
-            const int nwr = NC - nDB, wr = wave - nDB;
-            if constexpr (ELA) {
-                // four rows per wave at a time, 16 lanes each. A row is 9 len doubles
-                // (~135) moved as 16-byte pieces, one 8-byte piece at the end of an
-                // odd-length row (8-byte aligned 16-byte stores are valid on gfx950; the
-                // LDS side reads two 8-byte words). A lane reads all its pieces of a pass
-                // (CP of them, 32 CP doubles per row and pass) before it stores any, so
-                // a row costs one LDS round trip, not one per piece.
-                constexpr int CP = 5;
-                const int g = lane >> 4, l16 = lane & 15;
-                for (int rr = wr * 4 + g; rr < R; rr += nwr * 4) {
-                    const int4 ri = lri[rr];
-                    const int n = (ri.y & 0xFFFF) * 9;
-                    const double *src = outv + (int64_t)ri.w * 9;
-                    double *dst = values + (int64_t)ri.x * 9;
-                    for (int j0 = 2 * l16; j0 < n; j0 += 32 * CP) {
-                        double2 v[CP];
-#pragma unroll
-                        for (int k = 0; k < CP; k++) {
-                            const int j = j0 + 32 * k;
-                            v[k] = make_double2(j < n ? src[j] : 0.0, j + 1 < n ? src[j + 1] : 0.0);
-                        }
-#pragma unroll
-                        for (int k = 0; k < CP; k++) {
-                            const int j = j0 + 32 * k;
-                            if (j + 1 < n)
-                                *reinterpret_cast<double2 *>(dst + j) = v[k];
-                            else if (j < n)
-                                dst[j] = v[k].x;
-                        }
-                    }
-                }
-            } else {
-                // four rows per wave, 16 lanes each
-                const int g = lane >> 4, l16 = lane & 15;
-                for (int rr = wr * 4 + g; rr < R; rr += nwr * 4) {
-                    const int4 ri = lri[rr];
-                    const int n = ri.y & 0xFFFF;
-                    for (int j = l16; j < n; j += 16) values[(int64_t)ri.x + j] = outv[ri.w + j];
-                }
-            }
-        }
-        // the next tile's vertices, coordinates and coefficients are written meanwhile
-        __syncthreads();  // 4
+        const TileJob &jb = J.at(jcur);
+        if (is_ela(jb)) compute_tile<OPS != MF_OP_LAP, PREC, BT, FRONT_ELA>(h, jb, f_node, f_elem, lds_raw);
+        else compute_tile<OPS == MF_OP_ELA, PREC, BT, FRONT_ELA>(h, jb, f_node, f_elem, lds_raw);
         const int nslot = slot + G;
-        if (!(nslot < per && xcd * per + nslot < ntiles)) break;
+        if (nslot >= nslots) break;
         slot = nslot;
-        h = tiles[xcd * per + slot];
+        const int t = seek(W, cur, slot);
+        jcur = cur.j;
+        h = load_hdr(J.at(jcur).tiles, t);
     }
+}
+
+template <int OP, int PREC, int BT>
+__global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
+    k_tiled(TileJob job, int G, int f_node, int f_elem)
+{
+    tiled_body<OP, PREC, BT>(OneJob{job}, G, f_node, f_elem);
+}
+
+// The tiles of up to kMaxJobs plans, lap and ela mixed, in one persistent launch.
+template <int PREC, int BT>
+__global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4)))
+    k_tiled_jobs(JobTable table, int G, int f_node, int f_elem)
+{
+    tiled_body<kOpsAny, PREC, BT>(ManyJobs{table}, G, f_node, f_elem);
 }
 
 // Nodes with an empty row: prec_init leaves zeros; lap inverts them to inf, ela masks
@@ -762,6 +962,27 @@ __global__ void __launch_bounds__(1024)
     }
 }
 
+TileJob make_job(const mf_plan *p, const double *coord, const double *coef, double *values,
+                 double *prec, int *singular)
+{
+    TileJob j{};
+    j.tiles = p->tiles;
+    j.tile_elems = p->tile_elems;
+    j.tile_verts = p->tile_verts;
+    j.tile_nodes = p->tile_nodes;
+    j.rowinfo = p->tile_rowinfo;
+    j.inc8 = reinterpret_cast<const uint4 *>(p->inc);
+    j.task_rec = p->task_rec;
+    j.coord = coord;
+    j.coef = coef;
+    j.values = values;
+    j.prec = prec;
+    j.singular = singular;
+    j.op = p->op;
+    j.ntiles = p->ntiles;
+    return j;
+}
+
 template <typename T>
 int upload(T **dst, const std::vector<T> &src)
 {
@@ -828,11 +1049,9 @@ int launch_tiled(mf_plan *p, const double *coord, const double *coef, double *va
     if (p->ntiles > 0) {
         const int G = p->grid[PREC];
         if (G < 1) return MF_ERR_STATE;  // not prepared at creation
-        hipLaunchKernelGGL(kern, dim3(G * kXcd), dim3(BT), lds, s, p->tiles, p->ntiles, G,
-                           p->max_node, p->max_elem,
-                           p->tile_elems, p->tile_verts, p->tile_nodes, p->tile_rowinfo,
-                           reinterpret_cast<const uint4 *>(p->inc), p->task_rec, coord, coef,
-                           values, prec, singular);
+        hipLaunchKernelGGL(kern, dim3(G * kXcd), dim3(BT), lds, s,
+                           make_job(p, coord, coef, values, prec, singular), G, p->max_node,
+                           p->max_elem);
         MF_CHECK_LAUNCH();
     }
     if (p->n_isolated > 0) {
@@ -901,25 +1120,32 @@ int resolve_params(int op, const mf_plan_params *params, Resolved *r)
 
 int build_checked(mf::PlanHost &h, int op, int nE, int N, int nnz, const int *e2n,
                   const int *row, const int *col, const int *cb, const double *coord,
-                  const Resolved &r, int *lds64, int *lds32)
+                  const Resolved &r, int *lds64, int *lds32, int *back64 = nullptr,
+                  int *back32 = nullptr)
 {
     int st = mf::build_plan(h, op, nE, N, nnz, e2n, row, col, cb, coord, r.budget, r.max_rows,
                             r.slot64, r.block, r.id_chunks);
     if (st != MF_OK) return st;
     st = mf::validate_plan(h, row, col, e2n, r.slot64);
     if (st != MF_OK) return st;
-    int m64 = 0, m32 = 0;
+    int m64 = 0, m32 = 0, b64 = 0, b32 = 0;
     for (const auto &t : h.tiles) {
         // the front part (fixed offsets) is sized by the largest tile
         const bool ela = op == MF_OP_ELA;
-        m64 = std::max(m64, mf::tile_lds_layout(ela, h.max_node, h.max_elem, t.n_elem, t.n_rows,
-                                                t.n_inc, t.n_task, t.n_ent, r.slot64).total);
-        m32 = std::max(m32, mf::tile_lds_layout(ela, h.max_node, h.max_elem, t.n_elem, t.n_rows,
-                                                t.n_inc, t.n_task, t.n_ent, r.slot32).total);
+        const mf::TileLds L64 = mf::tile_lds_layout(ela, h.max_node, h.max_elem, t.n_elem,
+                                                    t.n_rows, t.n_inc, t.n_task, t.n_ent, r.slot64);
+        const mf::TileLds L32 = mf::tile_lds_layout(ela, h.max_node, h.max_elem, t.n_elem,
+                                                    t.n_rows, t.n_inc, t.n_task, t.n_ent, r.slot32);
+        m64 = std::max(m64, L64.total);
+        m32 = std::max(m32, L32.total);
+        b64 = std::max(b64, L64.total - L64.stage);
+        b32 = std::max(b32, L32.total - L32.stage);
     }
     if (m64 + 16 > mf::kMaxLdsBytes) return MF_ERR_ARG;  // one row needs more than a CU has
     *lds64 = m64 + 16;
     *lds32 = m32 + 16;
+    if (back64) *back64 = b64;
+    if (back32) *back32 = b32;
     return MF_OK;
 }
 
@@ -961,9 +1187,9 @@ extern "C" int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nb
         int st = resolve_params(operatorID, params, &r);
         if (st != MF_OK) return st;
         mf::PlanHost h;
-        int lds64 = 0, lds32 = 0;
+        int lds64 = 0, lds32 = 0, back64 = 0, back32 = 0;
         st = build_checked(h, operatorID, nbElem, nbNodes, nnz, elemToNode, row, col, checkBounds,
-                           coord, r, &lds64, &lds32);
+                           coord, r, &lds64, &lds32, &back64, &back32);
         if (st != MF_OK) return st;
 
         // every host container is built before the first device allocation, and the plan
@@ -983,6 +1209,8 @@ extern "C" int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nb
         p->block = r.block;
         p->lds_fp64 = lds64;
         p->lds_mixed = lds32;
+        p->back_fp64 = back64;
+        p->back_mixed = back32;
         p->n_isolated = (int)h.isolated.size();
         p->max_node = h.max_node;
         p->max_elem = h.max_elem;
@@ -1080,4 +1308,135 @@ extern "C" int mf_plan_assemble(mf_plan *plan, const double *coord, const double
     default:
         return MF_ERR_ARG;
     }
+}
+
+namespace {
+
+// LDS of a launch over several plans: the fixed front sized by their largest tiles with
+// ela's coefficient region (k_tiled_jobs keeps one front layout for both operators),
+// then the largest back part of any of their tiles.
+int jobs_lds(int f_node, int f_elem, int back)
+{
+    const int verts = mf::align16(f_node * 24);
+    const int coef = mf::align16(verts + (f_elem + 1) * 4);
+    return coef + mf::align16(f_elem * 16) + back + 16;
+}
+
+// Persistent grid of k_tiled_jobs for `lds` bytes of LDS per workgroup: as many
+// workgroups per XCD as fit at once, at most one per slot of an XCD's tile range.
+// Occupancy is asked once per kernel and LDS size (not a stream operation).
+std::mutex g_jobs_mu;
+std::map<std::pair<const void *, int>, int> g_jobs_per_cu;
+int g_jobs_cus = 0;
+
+template <int PREC, int BT>
+int jobs_grid(int lds, int slots, int *G)
+{
+    const void *kern = reinterpret_cast<const void *>(k_tiled_jobs<PREC, BT>);
+    std::lock_guard<std::mutex> lock(g_jobs_mu);
+    auto it = g_jobs_per_cu.find({kern, lds});
+    if (it == g_jobs_per_cu.end()) {
+        int dev = 0, per_cu = 0;
+        if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                mf::kMaxLdsBytes) != hipSuccess ||
+            hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&g_jobs_cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BT, lds) != hipSuccess)
+            return MF_ERR_HIP;
+        it = g_jobs_per_cu.emplace(std::make_pair(kern, lds), std::max(1, per_cu)).first;
+    }
+    *G = std::max(1, std::min(slots, it->second * std::max(1, g_jobs_cus / kXcd)));
+    return MF_OK;
+}
+
+struct JobsLaunch {
+    JobTable table{};
+    int ntiles = 0, slots = 0, f_node = 0, f_elem = 0, back = 0, block = 0, lds = 0;
+};
+
+int resolve_jobs(const mf_plan_job *jobs, int njobs, int precision, JobsLaunch &L)
+{
+    if (!jobs || njobs < 1 || njobs > kMaxJobs) return MF_ERR_ARG;
+    if (precision != MF_PREC_FP64 && precision != MF_PREC_MIXED &&
+        precision != MF_PREC_MIXED_F32COEF)
+        return MF_ERR_ARG;
+    for (int i = 0; i < njobs; i++) {
+        const mf_plan_job &jb = jobs[i];
+        const mf_plan *p = jb.plan;
+        if (!p || !jb.values || !jb.prec || (p->N > 0 && !jb.coord)) return MF_ERR_ARG;
+        if (L.block == 0) L.block = p->block;
+        if (p->block != L.block) return MF_ERR_ARG;  // one workgroup shape per launch
+        L.table.job[i] = make_job(p, jb.coord, jb.coef, jb.values, jb.prec, jb.singular);
+        L.ntiles += p->ntiles;
+        L.slots += (p->ntiles + kXcd - 1) / kXcd;  // the longest XCD walk holds this many
+        if (p->ntiles > 0) {
+            L.f_node = std::max(L.f_node, p->max_node);
+            L.f_elem = std::max(L.f_elem, p->max_elem);
+            L.back = std::max(L.back, precision == MF_PREC_FP64 ? p->back_fp64 : p->back_mixed);
+        }
+    }
+    L.table.njobs = njobs;
+    L.lds = L.ntiles > 0 ? jobs_lds(L.f_node, L.f_elem, L.back) : 0;
+    return L.lds > mf::kMaxLdsBytes ? MF_ERR_ARG : MF_OK;
+}
+
+template <int PREC, int BT>
+int launch_jobs(const JobsLaunch &L, hipStream_t s)
+{
+    int G = 0;
+    const int st = jobs_grid<PREC, BT>(L.lds, L.slots, &G);
+    if (st != MF_OK) return st;
+    auto kern = k_tiled_jobs<PREC, BT>;
+    hipLaunchKernelGGL(kern, dim3(G * kXcd), dim3(BT), L.lds, s, L.table, G, L.f_node, L.f_elem);
+    MF_CHECK_LAUNCH();
+    return MF_OK;
+}
+
+template <int PREC>
+int launch_jobs_variant(const JobsLaunch &L, hipStream_t s)
+{
+    switch (L.block) {
+    case 512: return launch_jobs<PREC, 512>(L, s);
+    case 1024: return launch_jobs<PREC, 1024>(L, s);
+    default: return MF_ERR_ARG;
+    }
+}
+
+}  // namespace
+
+extern "C" int mf_plans_lds_bytes(const mf_plan_job *jobs, int njobs, int precision)
+{
+    JobsLaunch L;
+    const int st = resolve_jobs(jobs, njobs, precision, L);
+    return st != MF_OK ? st : L.lds;
+}
+
+extern "C" int mf_plans_assemble(const mf_plan_job *jobs, int njobs, int precision, void *stream)
+{
+    JobsLaunch L;
+    int st = resolve_jobs(jobs, njobs, precision, L);
+    if (st != MF_OK) return st;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (L.ntiles > 0) {
+        st = precision == MF_PREC_FP64    ? launch_jobs_variant<MF_PREC_FP64>(L, s)
+             : precision == MF_PREC_MIXED ? launch_jobs_variant<MF_PREC_MIXED>(L, s)
+                                          : launch_jobs_variant<MF_PREC_MIXED_F32COEF>(L, s);
+        if (st != MF_OK) return st;
+    }
+    // nodes without elements: the plans that have them add their own small launch
+    for (int i = 0; i < njobs; i++) {
+        const mf_plan *p = jobs[i].plan;
+        if (p->n_isolated == 0) continue;
+        if (p->op == MF_OP_ELA)
+            hipLaunchKernelGGL(k_prec_isolated<MF_OP_ELA>, dim3((p->n_isolated + 255) / 256),
+                               dim3(256), 0, s, p->isolated, p->n_isolated, p->cb, p->N,
+                               jobs[i].prec);
+        else
+            hipLaunchKernelGGL(k_prec_isolated<MF_OP_LAP>, dim3((p->n_isolated + 255) / 256),
+                               dim3(256), 0, s, p->isolated, p->n_isolated, p->cb, p->N,
+                               jobs[i].prec);
+        MF_CHECK_LAUNCH();
+    }
+    return MF_OK;
 }

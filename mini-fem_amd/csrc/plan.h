@@ -120,15 +120,17 @@ struct TileLds {
     int coords, verts, coef, stage, rinfo, tasks, rnode, inc, total;
 };
 
-MF_HOST_DEVICE inline TileLds tile_lds_layout(bool ela, int f_node, int f_elem, int n_elem,
-                                              int n_rows, int n_inc, int n_task, int n_ent,
-                                              int slot_bytes)
+// front_ela: the front's coefficient region holds ela's 16 bytes per element (a launch
+// that mixes operators uses it for every tile); ela: the operator of the tile itself.
+MF_HOST_DEVICE inline TileLds tile_lds_layout(bool front_ela, bool ela, int f_node, int f_elem,
+                                              int n_elem, int n_rows, int n_inc, int n_task,
+                                              int n_ent, int slot_bytes)
 {
     TileLds L;
     L.coords = 0;
     L.verts = align16(f_node * 24);
     L.coef = align16(L.verts + (f_elem + 1) * 4);
-    L.stage = L.coef + align16(f_elem * (ela ? 16 : 8));
+    L.stage = L.coef + align16(f_elem * (front_ela ? 16 : 8));
     const int stage = (n_elem + 1) * slot_bytes, out = n_ent * (ela ? 72 : 8);
     L.rinfo = L.stage + align16(stage > out ? stage : out);
     L.tasks = L.rinfo + n_rows * 16;
@@ -136,6 +138,15 @@ MF_HOST_DEVICE inline TileLds tile_lds_layout(bool ela, int f_node, int f_elem, 
     L.inc = align16(L.rnode + n_rows * 4);
     L.total = L.inc + align16(n_inc * 2);
     return L;
+}
+
+// The layout of a launch of one operator (the front sized for that operator).
+MF_HOST_DEVICE inline TileLds tile_lds_layout(bool ela, int f_node, int f_elem, int n_elem,
+                                              int n_rows, int n_inc, int n_task, int n_ent,
+                                              int slot_bytes)
+{
+    return tile_lds_layout(ela, ela, f_node, f_elem, n_elem, n_rows, n_inc, n_task, n_ent,
+                           slot_bytes);
 }
 
 // Diagonal work blocks of a tile: one row per lane.

@@ -74,6 +74,7 @@ SIGNATURES = {
     "mf_event_record": (_i, [_p, _p]),
     "mf_event_elapsed_ms": (_i, [_p, _p, _p]),
     "mf_hbm_copy": (_i, [_p, _p, ctypes.c_size_t, _p]),
+    "mf_gather_probe": (_i, [_p, _p, _p, _i64, _p]),
     "mf_dev_create_nodeToElem": (_i, [_p, _i, _i, _p, _p, _p]),
     "mf_dev_create_nodeToNode": (_i, [_p, _p, _p, _p, _p, _i, _i, _p]),
     "mf_dev_create_elemToEdge": (_i, [_p, _p, _p, _p, _i, _p]),
@@ -94,7 +95,17 @@ SIGNATURES = {
     "mf_plan_assemble": (_i, [_p, _p, _p, _p, _p, _i, _p, _p]),
     "mf_plan_write_probe": (_i, [_p, _p, _i, _p]),
     "mf_plan_host_stats": (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p]),
+    "mf_plans_assemble": (_i, [_p, _i, _i, _p]),
+    "mf_plans_lds_bytes": (_i, [_p, _i, _i]),
 }
+
+MAX_PLAN_JOBS = 16
+
+
+class PlanJob(ctypes.Structure):
+    """mf_plan_job: one plan's iteration inside a multi-plan launch."""
+    _fields_ = [("plan", _p), ("coord", _p), ("coef", _p), ("values", _p), ("prec", _p),
+                ("singular", _p)]
 
 
 class PlanParams(ctypes.Structure):

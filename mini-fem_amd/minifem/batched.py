@@ -7,9 +7,10 @@ Two forms:
     round-robin to `branches` parallel streams inside the graph (forked from and joined
     back to the capturing stream through events);
   - union (union=True): the meshes of one operator become ONE disjoint-union mesh with
-    one plan, so the whole batch is one persistent launch per operator (two side streams
-    when both operators are present). The workgroups of that launch take tiles of every
-    mesh, so small meshes never leave the GPU idle and no launch waits for another.
+    one plan, and the plans of both operators run as ONE persistent launch
+    (mf_plans_assemble: the tiles of the lap union, then those of the ela union, each tile
+    running its own operator). The workgroups of that launch take tiles of every mesh, so
+    small meshes never leave the GPU idle and no launch waits for another.
     Results are bit-identical to the per-mesh form: the greedy coloring of a disjoint
     union colors every element as its own mesh does, and its color permutation keeps
     each mesh's elements in their own relative order, so every CSR block sums the same
@@ -19,7 +20,7 @@ The schedule parameters of the tiled plans are picked for the whole batch by `tu
 import numpy as np
 
 from . import gpu
-from ._lib import OP_ELA, OP_LAP, PREC_FP64
+from ._lib import OP_ELA, OP_LAP, PREC_FP64, PREC_MIXED
 from .setup import Mesh, generate_coef, generate_cube
 
 
@@ -103,6 +104,7 @@ class Batch:
         self.precision = precision
         self.union = bool(union)
         self._where = None
+        self._jobs = None  # union form: the job table of its one launch
         if self.union:
             # One union per operator, in order of first appearance; _where[i] = (union,
             # part) of input mesh i.
@@ -115,8 +117,7 @@ class Batch:
                 for k, i in enumerate(idx):
                     self._where[i] = (u, k)
             self.dms = [gpu.DeviceMesh(u.mesh) for u in self.unions]
-            self.plans = [gpu.Plan(u.mesh, partition_coord=u.partition_coord, **plan_params)
-                          for u in self.unions]
+            self.plans = self._union_plans(plan_params)
         else:
             self.dms = [gpu.DeviceMesh(m) for m in self.meshes]
             self.plans = [gpu.Plan(m, **plan_params) for m in self.meshes]
@@ -127,6 +128,23 @@ class Batch:
         self._join = [gpu.Event() for _ in range(self.branches)]
         self.graph = None
 
+    def _union_plans(self, plan_params):
+        """The union plans, sized so that they run as one launch: when their largest tiles
+        do not fit one CU's LDS together, they are rebuilt with the tile budget lowered by
+        the overflow."""
+        params = dict(plan_params)
+        for _ in range(3):
+            plans = [gpu.Plan(u.mesh, partition_coord=u.partition_coord, **params)
+                     for u in self.unions]
+            jobs = [(p, dm.coord, dm.coef, dm.values, dm.prec) for p, dm in zip(plans, self.dms)]
+            fits = [gpu.plans_lds_bytes(jobs, prec) for prec in (PREC_FP64, PREC_MIXED)]
+            if all(f is not None for f in fits):
+                self._jobs = jobs
+                return plans
+            budget = params.get("tile_lds_bytes") or 155000
+            params["tile_lds_bytes"] = budget - 6144  # the front of the other operator's tiles
+        raise RuntimeError("union plans do not fit one launch")
+
     def set_precision(self, precision):
         """Switches the precision mode of every launch (the plans serve every mode); the
         captured graph, if any, is dropped and captured again on the next launch()."""
@@ -134,7 +152,11 @@ class Batch:
         self.graph = None
 
     def enqueue(self, stream):
-        """Every mesh's iteration, forked over the side streams and joined on `stream`."""
+        """Every mesh's iteration: the union form's one launch on `stream`, or the per-mesh
+        launches forked over the side streams and joined on `stream`."""
+        if self._jobs is not None:
+            gpu.plans_assemble(self._jobs, self.precision, stream)
+            return
         self._fork.record(stream)
         for s in self._side:
             s.wait(self._fork)
@@ -181,7 +203,7 @@ class Batch:
 
     def launches(self):
         """Assembly launches per batch iteration."""
-        return len(self.plans)
+        return 1 if self._jobs is not None else len(self.plans)
 
     def results(self):
         """(values, prec) of every input mesh, in input order."""
@@ -196,8 +218,11 @@ class Batch:
 
 
 BATCH_CANDIDATES = [
-    dict(tile_lds_bytes=lds, block_threads=bt, max_tile_rows=254)
-    for lds, bt in ((40000, 512), (64000, 512), (80000, 512), (150000, 1024))
+    dict(tile_lds_bytes=lds, block_threads=bt, max_tile_rows=rows)
+    for lds, bt, rows in ((40000, 512, 254), (56000, 512, 254), (64000, 512, 254),
+                          (72000, 512, 254), (80000, 512, 254), (100000, 1024, 254),
+                          (120000, 1024, 254), (135000, 1024, 254), (150000, 1024, 254),
+                          (150000, 1024, 48))
 ]
 
 

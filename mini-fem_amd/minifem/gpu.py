@@ -322,6 +322,37 @@ class DeviceMesh:
                        stream=stream)
 
 
+def _job_table(jobs):
+    from ._lib import MAX_PLAN_JOBS, PlanJob
+    jobs = list(jobs)
+    if not 1 <= len(jobs) <= MAX_PLAN_JOBS:
+        raise ValueError(f"a multi-plan launch takes 1..{MAX_PLAN_JOBS} jobs, got {len(jobs)}")
+    arr = (PlanJob * len(jobs))()
+    for k, j in enumerate(jobs):
+        plan, coord, coef, values, prec = j[:5]
+        singular = j[5] if len(j) > 5 else None
+        arr[k] = PlanJob(plan.ptr, _p(coord), _p(coef), _p(values), _p(prec), _p(singular))
+    return arr, len(jobs)
+
+
+def plans_assemble(jobs, precision=PREC_FP64, stream=None):
+    """One persistent launch over the tiles of several plans (mf_plans_assemble), lap and
+    ela mixed. jobs: (plan, coord, coef, values, prec[, singular]) per plan, the arguments
+    Plan.assemble takes; each job's results equal its own Plan.assemble bit for bit."""
+    arr, n = _job_table(jobs)
+    check(lib().mf_plans_assemble(ctypes.cast(arr, ctypes.c_void_p), n, int(precision),
+                                  _sp(stream)), "mf_plans_assemble")
+
+
+def plans_lds_bytes(jobs, precision=PREC_FP64):
+    """LDS bytes per workgroup of that launch, or None when the jobs do not fit one CU."""
+    arr, n = _job_table(jobs)
+    r = lib().mf_plans_lds_bytes(ctypes.cast(arr, ctypes.c_void_p), n, int(precision))
+    if r == -1:
+        return None
+    return check(r, "mf_plans_lds_bytes")
+
+
 class Plan:
     """A mesh topology resident on the GPU with its tiled gather schedule
     (mf_plan_create). assemble() runs one fused iteration: values + inverted prec."""

@@ -6,28 +6,30 @@ per-element coefficient field or perturbed coordinates. Serving it means uploadi
 input, running one fused iteration (assembly + block-Jacobi prec, src/FEM.cc:183-232)
 and leaving values and prec in HBM for the consumer.
 
-Admission is FIFO. A request starts once it has arrived, a slot (stream) is free, and the
-HBM budget can hold its footprint: input + values + prec. The budget is managed as an arena
-carved from one device allocation, so serving itself makes no hipMalloc/hipFree calls
-(hipFree would synchronise the device).
+Admission: when a slot (stream) is free, the requests that have arrived and whose
+footprint (input + values + prec) the HBM budget can hold are taken in arrival order, up to
+max_group of them with one precision mode, and served by ONE launch over all their plans
+(mf_plans_assemble: each tile runs its own mesh and operator). A request that does not fit
+waits; later requests may pass it (first fit) until it has been passed max_bypass times,
+then admission holds until it fits. The budget is managed as an arena carved from one
+device allocation, so serving itself makes no hipMalloc/hipFree calls (hipFree would
+synchronise the device).
 
-Requests in flight on different slots overlap on the device. The loop polls completion
-events and frees memory and slots as requests finish, so new arrivals are admitted
-while others still run. Latency is measured from arrival to completion of the request's
-last kernel.
+Launches on different slots overlap on the device. The loop polls completion events and
+frees memory and slots as launches finish, so new arrivals are admitted while others still
+run. Latency is measured from arrival to completion of the launch that served the request.
 
 The mixed-precision server takes a request's coefficient field as float32, the precision
 its element products use (coordinates stay float64: their differences form the element
 Jacobian), which halves those uploads.
 """
 import bisect
-import collections
 import time
 
 import numpy as np
 
 from . import gpu
-from ._lib import OP_ELA, PREC_FP64, PREC_MIXED, PREC_MIXED_F32COEF
+from ._lib import MAX_PLAN_JOBS, OP_ELA, PREC_FP64, PREC_MIXED, PREC_MIXED_F32COEF
 from .setup import Mesh, generate_coef, generate_cube
 
 _ALIGN = 256
@@ -116,26 +118,60 @@ def make_requests(meshes, count, rate_hz, seed=5000):
     return reqs
 
 
-class Server:
-    """Resident meshes + an HBM budget + `slots` concurrent request streams."""
+def retime(requests, rate_hz, seed=5000):
+    """The same requests (meshes, payloads, staged buffers) with Poisson arrivals at
+    another rate: one staging serves streams of several rates."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate_hz, len(requests)))
+    out = []
+    for r, ta in zip(requests, t):
+        q = Request(r.rid, float(ta), r.mesh, r.kind, r.payload)
+        q.host = r.host
+        out.append(q)
+    return out
 
-    def __init__(self, meshes, hbm_budget, slots=4, precision=PREC_FP64, plan_params=None):
+
+class Server:
+    """Resident meshes + an HBM budget + `slots` concurrent request streams; each launch
+    serves up to `max_group` requests at once."""
+
+    def __init__(self, meshes, hbm_budget, slots=4, precision=PREC_FP64, plan_params=None,
+                 max_group=8, max_bypass=8):
         if precision not in (PREC_FP64, PREC_MIXED):
             # MF_PREC_MIXED_F32COEF is chosen per request, for a float32 coefficient
             # payload staged by a PREC_MIXED server; a float64 field read as float is wrong
             raise ValueError(f"Server precision must be PREC_FP64 or PREC_MIXED, got {precision}")
         self.meshes = list(meshes)
         self.precision = precision
+        self.max_group = max(1, min(int(max_group), MAX_PLAN_JOBS))
+        self.max_bypass = max(0, int(max_bypass))
         self.dms = [gpu.DeviceMesh(m) for m in self.meshes]
-        self.plans = [gpu.Plan(m, **(plan_params or {})) for m in self.meshes]
+        self._plan_params = dict(plan_params or {})
+        self.plans = [gpu.Plan(m, **self._plan_params) for m in self.meshes]
         self.arena_mem = gpu.DeviceArray(int(hbm_budget), np.uint8)
         self.arena = Arena(hbm_budget)
         self.slots = [gpu.Stream() for _ in range(max(1, int(slots)))]
         self.events = [gpu.Event() for _ in self.slots]
+        self._fits = {}  # (precision, sorted plan indices) -> one launch fits one CU's LDS
         # one warm launch per plan before any timing (code objects loaded, caches warm)
         for p, dm in zip(self.plans, self.dms):
             p.assemble(dm.coord, dm.coef, dm.values, dm.prec, precision=precision)
         gpu.synchronize()
+
+    def add_mesh(self, mesh):
+        """Admits a new resident mesh while the server runs (its topology arrives with a
+        request stream of its own): device arrays, tiled plan and one warm launch on a slot
+        stream. The mesh's host setup (coloring, CSR, mask: minifem.Mesh) comes first.
+        Returns the mesh's index for requests."""
+        dm = gpu.DeviceMesh(mesh)
+        plan = gpu.Plan(mesh, **self._plan_params)
+        s = self.slots[0]
+        plan.assemble(dm.coord, dm.coef, dm.values, dm.prec, precision=self.precision, stream=s)
+        s.sync()
+        self.meshes.append(mesh)
+        self.dms.append(dm)
+        self.plans.append(plan)
+        return len(self.meshes) - 1
 
     def _views(self, req, off):
         m = self.meshes[req.mesh]
@@ -152,17 +188,72 @@ class Server:
         a = lambda n: (n + _ALIGN - 1) // _ALIGN * _ALIGN
         return a(req.host.nbytes) + a(8 * m.dim * m.nnz) + a(8 * m.dim * m.nbNodes)
 
-    def _launch(self, req, slot, inp, values, prec):
-        s = self.slots[slot]
+    def _mode(self, req):
+        """The precision mode of a request's launch: a float32 coefficient field is read as
+        float (MF_PREC_MIXED_F32COEF); everything else runs in the server's precision."""
+        if req.kind == "coef" and req.host.dtype == np.float32:
+            return PREC_MIXED_F32COEF
+        return self.precision
+
+    def _job(self, req, views):
+        inp, values, prec = views
         dm = self.dms[req.mesh]
-        gpu.copy_h2d_async(inp, req.host, s)
         coord = inp if req.kind == "coord" else dm.coord
         coef = inp if req.kind == "coef" else dm.coef
-        precision = self.precision
-        if req.kind == "coef" and req.host.dtype == np.float32:
-            precision = PREC_MIXED_F32COEF
-        self.plans[req.mesh].assemble(coord, coef, values, prec, precision=precision, stream=s)
+        return (self.plans[req.mesh], coord, coef, values, prec)
+
+    def _group_fits(self, mode, mesh_ids):
+        """Whether the plans of these meshes run as one launch (their tiles fit one CU's LDS
+        together; the check reads only the plans)."""
+        key = (mode, tuple(sorted(set(mesh_ids))))
+        if key not in self._fits:
+            jobs = [(self.plans[i], self.dms[i].coord, None, self.dms[i].values, self.dms[i].prec)
+                    for i in key[1]]
+            self._fits[key] = gpu.plans_lds_bytes(jobs, mode) is not None
+        return self._fits[key]
+
+    def _launch(self, group, slot):
+        """One launch for a group of requests on one slot: their inputs uploaded on the slot's
+        stream, then one multi-plan launch over all of them."""
+        s = self.slots[slot]
+        jobs = []
+        for req, _, views in group:
+            gpu.copy_h2d_async(views[0], req.host, s)
+            jobs.append(self._job(req, views))
+        gpu.plans_assemble(jobs, self._mode(group[0][0]), stream=s)
         self.events[slot].record(s)
+
+    def _take(self, pending, now, memory_waits):
+        """Admission for one free slot: requests that have arrived, in arrival order, that fit
+        the HBM budget, one precision mode per launch, at most max_group of them. A request
+        that does not fit waits; later ones may pass it (first fit) until it has been passed
+        max_bypass times, then admission holds until it fits."""
+        group, mode, i = [], None, 0
+        head = pending[0] if pending else None
+        while i < len(pending) and len(group) < self.max_group:
+            req = pending[i]
+            if req.t_arrival > now:
+                break
+            m = self._mode(req)
+            if mode is not None and (m != mode or not self._group_fits(
+                    m, [g[0].mesh for g in group] + [req.mesh])):
+                i += 1
+                continue
+            blk = self.arena.alloc(self._footprint_aligned(req))
+            if blk is None:
+                memory_waits.add(req.rid)
+                if req is head and self._bypassed >= self.max_bypass:
+                    break  # the head has waited long enough: nothing passes it
+                i += 1
+                continue
+            del pending[i]
+            group.append((req, blk, self._views(req, blk[0])))
+            mode = m
+        if head is not None and group and pending and pending[0] is head:
+            self._bypassed += 1  # the head stays behind the requests just admitted
+        elif not pending or pending[0] is not head:
+            self._bypassed = 0
+        return group
 
     def stage(self, requests):
         """Copies each request's payload into pinned host memory: the receive buffer a
@@ -181,42 +272,38 @@ class Server:
                 raise ValueError("stage() the requests first")
             if self._footprint_aligned(r) > self.arena.size:
                 raise ValueError(f"request {r.rid} needs more than the HBM budget")
-        pending = collections.deque(sorted(requests, key=lambda r: r.t_arrival))
-        busy = [None] * len(self.slots)  # (request, arena block, views)
+        pending = sorted(requests, key=lambda r: r.t_arrival)
+        busy = [None] * len(self.slots)  # per slot: [(request, arena block, views), ...]
         results = {}
         done = []
         memory_waits = set()  # requests that found a slot free but not their HBM
+        launches = 0
+        self._bypassed = 0
         t0 = time.perf_counter()
         while pending or any(b is not None for b in busy):
-            now = time.perf_counter() - t0
-            progressed = False
             for i, b in enumerate(busy):
                 if b is not None and self.events[i].query():
-                    req, blk, views = b
-                    req.t_done = time.perf_counter() - t0
-                    if keep_results:
-                        results[req.rid] = (views[1].numpy(), views[2].numpy())
-                    self.arena.free(blk)
+                    t_done = time.perf_counter() - t0
+                    for req, blk, views in b:
+                        req.t_done = t_done
+                        if keep_results:
+                            results[req.rid] = (views[1].numpy(), views[2].numpy())
+                        self.arena.free(blk)
+                        done.append(req)
                     busy[i] = None
-                    done.append(req)
-                    progressed = True
-            while pending and pending[0].t_arrival <= now:
-                free = [i for i, b in enumerate(busy) if b is None]
-                if not free:
-                    break
-                req = pending[0]
-                blk = self.arena.alloc(self._footprint_aligned(req))
-                if blk is None:
-                    memory_waits.add(req.rid)
-                    break  # FIFO: wait for memory to free up
-                pending.popleft()
-                views = self._views(req, blk[0])
-                req.t_start = time.perf_counter() - t0
-                self._launch(req, free[0], *views)
-                busy[free[0]] = (req, blk, views)
-                progressed = True
-            if not progressed:
-                time.sleep(0.00005)
+            now = time.perf_counter() - t0
+            for i in range(len(self.slots)):
+                if busy[i] is not None or not pending or pending[0].t_arrival > now:
+                    continue
+                group = self._take(pending, now, memory_waits)
+                if not group:
+                    break  # nothing admissible until memory frees up or more arrives
+                for req, _, _ in group:
+                    req.t_start = now
+                self._launch(group, i)
+                busy[i] = group
+                launches += 1
+            time.sleep(0)  # yield, then poll again
         makespan = max(r.t_done for r in done) - min(r.t_arrival for r in done)
         lat = np.array([r.t_done - r.t_arrival for r in done])
         blocks = sum(self.meshes[r.mesh].nnz for r in done)
@@ -225,16 +312,21 @@ class Server:
                  "mean_ms": float(lat.mean() * 1e3), "makespan_s": float(makespan),
                  "blocks_per_s": float(blocks / makespan),
                  "requests_per_s": float(len(done) / makespan),
-                 "memory_waits": len(memory_waits)}
+                 "memory_waits": len(memory_waits), "launches": launches,
+                 "mean_requests_per_launch": len(done) / max(launches, 1)}
         return (stats, results) if keep_results else stats
 
     def run_alone(self, req):
-        """One request with the device to itself: (values, prec)."""
+        """One request with the device to itself, through the single-plan launch
+        (mf_plan_assemble): (values, prec)."""
         blk = self.arena.alloc(self._footprint_aligned(req))
         try:
             views = self._views(req, blk[0])
-            self._launch(req, 0, *views)
-            self.slots[0].sync()
+            s = self.slots[0]
+            gpu.copy_h2d_async(views[0], req.host, s)
+            plan, coord, coef, values, prec = self._job(req, views)
+            plan.assemble(coord, coef, values, prec, precision=self._mode(req), stream=s)
+            s.sync()
             return views[1].numpy(), views[2].numpy()
         finally:
             self.arena.free(blk)

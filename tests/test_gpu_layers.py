@@ -62,7 +62,7 @@ def test_batched_union_equals_per_mesh(mf, gpu, precision):
     del ref
     for params in (dict(), dict(tile_lds_bytes=150000, block_threads=1024, max_tile_rows=254)):
         u = Batch(meshes, precision=precision, union=True, **params)
-        assert u.launches() == 2
+        assert u.launches() == 1
         u.capture()
         for rep in range(2):
             for dm in u.dms:
@@ -223,7 +223,7 @@ def test_batched_full_config(mf, orc, gpu):
             del got
             u.run_eager()
             gpu.synchronize()
-    print(f"[batched full] union: {u.launches()} launch(es) per batch", flush=True)
+    assert u.launches() == 1  # both operators' unions in one persistent launch
 
 
 @pytest.mark.parametrize("precision", [0, 1])
@@ -247,6 +247,7 @@ def test_served_full_config(mf, orc, gpu, precision):
     stats, mixed = srv.run(reqs, keep_results=True)
     print(f"[serve full] {stats}", flush=True)
     assert stats["requests"] == len(reqs) and srv.arena.used == 0
+    assert stats["launches"] < len(reqs)  # requests that arrived together shared launches
     assert 0 < stats["p50_ms"] <= stats["p99_ms"]
     for r in reqs:
         v, p = srv.run_alone(r)
@@ -275,3 +276,33 @@ def test_served_full_config(mf, orc, gpu, precision):
             ev = np.linalg.norm(mixed[r.rid][0] - v64) / np.linalg.norm(v64)
             ep = np.linalg.norm(mixed[r.rid][1] - p64) / np.linalg.norm(p64)
             assert ev <= 1e-6 and ep <= 1e-6, (r.rid, ev, ep)
+
+
+def test_served_new_mesh_admitted_at_run_time(mf, orc, gpu):
+    """A topology that was not resident joins a running server (Server.add_mesh): its
+    requests, mixed with requests on the original meshes in one stream, equal the same
+    requests served alone, and one of them equals the oracle."""
+    from minifem.serve import Server, make_requests, served_meshes
+    print("[serve] new mesh at run time", flush=True)
+    meshes = served_meshes(count=4, n_lo=1000, n_hi=20000, seed=1000)
+    fp = max(8 * m.dim * (m.nnz + m.nbNodes) + 8 * max(m.coord.size, m.coef.size)
+             for m in meshes)
+    srv = Server(meshes, hbm_budget=int(3.0 * fp), slots=3)
+    new = mf.Mesh.cube(18, 0.25, 77, mf.OP_ELA, shuffle_seed=78, coef_seed=79)
+    k = srv.add_mesh(new)
+    assert k == 4 and len(srv.plans) == 5
+    reqs = make_requests(srv.meshes, count=30, rate_hz=20000.0, seed=5001)
+    assert any(r.mesh == k for r in reqs)
+    srv.stage(reqs)
+    stats, mixed = srv.run(reqs, keep_results=True)
+    assert stats["requests"] == len(reqs)
+    for r in reqs:
+        v, p = srv.run_alone(r)
+        assert np.array_equal(v, mixed[r.rid][0]) and np.array_equal(p, mixed[r.rid][1]), r.rid
+    r = next(r for r in reqs if r.mesh == k)
+    coord = r.payload if r.kind == "coord" else new.coord
+    coef = new.coef_original if r.kind == "coord" else new.unpermute_elements(r.payload)
+    om = orc.OracleMesh(coord, new.elemToNode_original, new.boundNodesCode, new.nnz, new.operatorID,
+                        coef=coef)
+    ov = om.assemble()
+    assert np.array_equal(mixed[r.rid][0], ov) and np.array_equal(mixed[r.rid][1], om.prec(ov)[0])
