@@ -407,6 +407,15 @@ def batched_leg(args):
     return out
 
 
+def make_server(meshes, budget, precision, slots):
+    """The served forms: slots=1 is the plain server (one request per launch, one launch at
+    a time); 4 slots is continuous batching (4 streams, requests that arrived together
+    served in pairs by one launch)."""
+    from minifem.serve import Server
+    return Server(meshes, budget, slots=slots, precision=precision,
+                  max_group=1 if slots == 1 else 2)
+
+
 def served_leg(args, rate=5000.0, count=600, saturating=100000.0):
     """The served workload in the default line: 16 resident meshes (1k-885k nodes), a
     seeded Poisson stream under an HBM budget of half the footprints' sum, served by
@@ -416,7 +425,7 @@ def served_leg(args, rate=5000.0, count=600, saturating=100000.0):
     runs from arrival to the end of the request's last kernel (the results stay in HBM for
     the consumer; readback is not included)."""
     import minifem
-    from minifem.serve import Server, make_requests, retime, served_meshes
+    from minifem.serve import make_requests, retime, served_meshes
     t0 = time.time()
     meshes = served_meshes()
     reqs = make_requests(meshes, count, rate)
@@ -432,9 +441,7 @@ def served_leg(args, rate=5000.0, count=600, saturating=100000.0):
     for name, prec, slots in (("plain_fp64_1_slot", minifem.PREC_FP64, 1),
                               ("batched_fp64_4_slots", minifem.PREC_FP64, 4),
                               ("batched_mixed_4_slots", minifem.PREC_MIXED, 4)):
-        # plain: one request per launch, one launch at a time; batched: up to 8 requests
-        # per launch on each of 4 streams
-        srv = Server(meshes, budget, slots=slots, precision=prec, max_group=1 if slots == 1 else 8)
+        srv = make_server(meshes, budget, prec, slots)
         srv.stage(reqs)
         st = srv.run(reqs)
         out[name] = {k: round(v, 4) for k, v in st.items()}
@@ -559,7 +566,7 @@ def bench_serve(args):
 
     import minifem
     import minifem.gpu as gpu
-    from minifem.serve import Server, make_requests, served_meshes
+    from minifem.serve import make_requests, served_meshes
     gpu.init(0)
     t0 = time.time()
     meshes = served_meshes()
@@ -577,7 +584,7 @@ def bench_serve(args):
     for name, prec, slots in (("plain_fp64_one_at_a_time", minifem.PREC_FP64, 1),
                               ("batched_fp64", minifem.PREC_FP64, 4),
                               ("batched_mixed", minifem.PREC_MIXED, 4)):
-        srv = Server(meshes, budget, slots=slots, precision=prec, max_group=1 if slots == 1 else 8)
+        srv = make_server(meshes, budget, prec, slots)
         srv.stage(reqs)
         st = srv.run(reqs)
         out[name] = {k: round(v, 4) for k, v in st.items()}

@@ -6,18 +6,30 @@ per-element coefficient field or perturbed coordinates. Serving it means uploadi
 input, running one fused iteration (assembly + block-Jacobi prec, src/FEM.cc:183-232)
 and leaving values and prec in HBM for the consumer.
 
-Admission: when a slot (stream) is free, the requests that have arrived and whose
-footprint (input + values + prec) the HBM budget can hold are taken in arrival order, up to
-max_group of them with one precision mode, and served by ONE launch over all their plans
-(mf_plans_assemble: each tile runs its own mesh and operator). A request that does not fit
-waits; later requests may pass it (first fit) until it has been passed max_bypass times,
-then admission holds until it fits. The budget is managed as an arena carved from one
-device allocation, so serving itself makes no hipMalloc/hipFree calls (hipFree would
-synchronise the device).
+Admission: a request that has arrived is admitted once its footprint (input + values +
+prec) fits the HBM budget, in arrival order. One that does not fit waits; later requests
+may pass it (first fit) until it has been passed max_bypass times, then admission holds
+until it fits. The budget is managed as an arena carved from one device allocation, so
+serving itself makes no hipMalloc/hipFree calls (hipFree would synchronise the device).
 
-Launches on different slots overlap on the device. The loop polls completion events and
-frees memory and slots as launches finish, so new arrivals are admitted while others still
-run. Latency is measured from arrival to completion of the launch that served the request.
+When a slot (stream) is free, the admitted requests are taken in arrival order, up to
+max_group of them with one precision mode, and served by ONE launch over all their plans
+(mf_plans_assemble: each tile runs its own mesh and operator), so requests that arrived
+together share a launch. The slot uploads their inputs on its stream, then launches; with
+several slots, one slot's uploads run under another's kernel. With one slot and
+max_group=1 this is the plain one-at-a-time server.
+
+The defaults (4 slots, pairs) come from a sweep on the served workload (16 meshes of
+1k-885k nodes, FP64 and mixed, saturating arrivals): FP64 serving moves about 42 GB/s of
+inputs over the host-to-device link (57 GB/s measured), and 3-4 slots keep it busy. Pairs
+cut the median latency under saturation by a fifth at the same throughput; groups of 4-8
+lower throughput (a merged launch runs every member at the largest member's LDS footprint
+and lasts as long as its longest job). Uploading on separate copy streams, with the
+compute stream waiting for them on the device or on the host, was slower in every form.
+
+The loop polls completion events and frees memory and streams as launches finish, so new
+arrivals are admitted while others still run. Latency is measured from arrival to
+completion of the launch that served the request.
 
 The mixed-precision server takes a request's coefficient field as float32, the precision
 its element products use (coordinates stay float64: their differences form the element
@@ -132,11 +144,11 @@ def retime(requests, rate_hz, seed=5000):
 
 
 class Server:
-    """Resident meshes + an HBM budget + `slots` concurrent request streams; each launch
-    serves up to `max_group` requests at once."""
+    """Resident meshes + an HBM budget + `slots` streams; each launch serves up to
+    `max_group` requests at once (slots=1, max_group=1: the plain one-at-a-time server)."""
 
     def __init__(self, meshes, hbm_budget, slots=4, precision=PREC_FP64, plan_params=None,
-                 max_group=8, max_bypass=8):
+                 max_group=2, max_bypass=8):
         if precision not in (PREC_FP64, PREC_MIXED):
             # MF_PREC_MIXED_F32COEF is chosen per request, for a float32 coefficient
             # payload staged by a PREC_MIXED server; a float64 field read as float is wrong

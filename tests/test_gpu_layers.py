@@ -138,14 +138,17 @@ def test_serving_mixed_mix_equals_alone_and_error(mf, orc, gpu):
     assert worst <= 1e-6
 
 
-def test_serving_mix_equals_alone(mf, orc, gpu):
+@pytest.mark.parametrize("max_group", [1, 2, 8])
+def test_serving_mix_equals_alone(mf, orc, gpu, max_group):
+    """Every result in the mix equals the request served alone, whether requests are served
+    one per launch or merged, and two equal the oracle."""
     from minifem.serve import Server, make_requests, served_meshes
-    print("[serve] mix vs alone", flush=True)
+    print(f"[serve] mix vs alone, max_group {max_group}", flush=True)
     meshes = served_meshes(count=6, n_lo=1000, n_hi=60000, seed=1000)
     reqs = make_requests(meshes, count=40, rate_hz=20000.0, seed=5000)
     fp = max(8 * m.dim * (m.nnz + m.nbNodes) + 8 * max(m.coord.size, m.coef.size)
              for m in meshes)
-    srv = Server(meshes, hbm_budget=int(2.2 * fp), slots=4)
+    srv = Server(meshes, hbm_budget=int(2.2 * fp), slots=4, max_group=max_group)
     srv.stage(reqs)
     stats, mixed = srv.run(reqs, keep_results=True)
     assert stats["requests"] == len(reqs)
