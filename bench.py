@@ -424,6 +424,8 @@ def served_leg(args, rate=5000.0, count=600, saturating=100000.0):
     requests at a saturating rate (throughput: every form busy all the time). Latency
     runs from arrival to the end of the request's last kernel (the results stay in HBM for
     the consumer; readback is not included)."""
+    import numpy as np
+
     import minifem
     from minifem.serve import make_requests, retime, served_meshes
     t0 = time.time()
@@ -459,18 +461,27 @@ def served_leg(args, rate=5000.0, count=600, saturating=100000.0):
         del got
         if name != "batched_mixed_4_slots":  # the last server also admits a new mesh
             del srv
-    # a topology that was not resident: host setup (coloring, CSR, mask), then the plan,
-    # device arrays and a warm launch, while the server holds its 16 meshes
+    # a topology that was not resident: its setup (coloring, permutation, CSR, mask) on the
+    # GPU, then the plan, device arrays and a warm launch, while the server holds its 16
+    # meshes; the same setup on the host is timed beside it
     from minifem.setup import Mesh, generate_coef, generate_cube
     coord, e2n, code, nnz = generate_cube(100, 0.2, 4242, 4243)
     coef = generate_coef(1, e2n.size // 4, 4244)
+    t0h = time.perf_counter()
+    hmesh = Mesh(coord, e2n, code, nnz, 1, coef=coef)
     t1 = time.perf_counter()
-    mesh = Mesh(coord, e2n, code, nnz, 1, coef=coef)
+    mesh = Mesh(coord, e2n, code, nnz, 1, coef=coef, setup="device")
     t2 = time.perf_counter()
+    same = all(np.array_equal(getattr(hmesh, a), getattr(mesh, a))
+               for a in ("elemToNode", "coef", "colorPerm", "colorToElem", "row", "col"))
+    del hmesh
     idx = srv.add_mesh(mesh)
     t3 = time.perf_counter()
     out["new_mesh_admission"] = {"mesh": f"ela 100^3 node-shuffled, {mesh.nbNodes} nodes, "
-                                         f"{mesh.nbElem} tets", "host_setup_s": round(t2 - t1, 3),
+                                         f"{mesh.nbElem} tets",
+                                 "device_setup_s": round(t2 - t1, 3),
+                                 "host_setup_s": round(t1 - t0h, 3),
+                                 "device_setup_equals_host": bool(same),
                                  "plan_upload_warm_s": round(t3 - t2, 3),
                                  "total_s": round(t3 - t1, 3), "mesh_index": idx}
     del srv, mesh

@@ -133,16 +133,28 @@ double mf_compute_double_norm(const double *tab, int64_t size);
 
 /* Device twins of the setup above, bit-exact with them, for meshes that are already
  * resident on the GPU (all arrays device memory; each call returns after its work is
- * done). Greedy first-fit coloring is a sequential scan and stays on the host.
+ * done).
  *   mf_dev_create_nodeToElem: index[N+1], value[4nE], each node's elements ascending.
  *   mf_dev_create_nodeToNode: row[N+1], col[nbEdges]; returns nnz or MF_ERR_NNZ.
- *   mf_dev_create_elemToEdge: elemToEdge[16 nE]. */
+ *   mf_dev_create_elemToEdge: elemToEdge[16 nE].
+ *   mf_dev_coloring_creation (src/coloring.cc:46-109): mf_coloring_creation's outputs (colorPart may be NULL;
+ *     colorToElem in device memory, nbColors a host pointer), from the elements in
+ *     dependency levels (an element is colored once every lower-index element sharing a
+ *     node with it is), so the colors equal the sequential scan's. elemToNode 16-byte
+ *     aligned (MF_ERR_ARG otherwise), as hipMalloc returns it. */
 int mf_dev_create_nodeToElem(const int *elemToNode, int nbElem, int nbNodes, int *index,
                              int *value, void *stream);
 int mf_dev_create_nodeToNode(int *row, int *col, const int *n2eIndex, const int *n2eValue,
                              const int *elemToNode, int nbNodes, int nbEdges, void *stream);
 int mf_dev_create_elemToEdge(const int *row, const int *col, const int *elemToNode,
                              int *elemToEdge, int nbElem, void *stream);
+int mf_dev_coloring_creation(const int *elemToNode, int *colorPart, int *colorPerm,
+                             int nbElem, int nbNodes, int *colorToElem, int *nbColors,
+                             void *stream);
+/* mf_permute_int_2d / mf_permute_double_2d on device arrays (MF_ERR_ARG, tab unchanged,
+ * when a perm entry is outside [0, n)). */
+int mf_dev_permute_int_2d(int *tab, const int *perm, int n, int dim, void *stream);
+int mf_dev_permute_double_2d(double *tab, const int *perm, int n, int dim, void *stream);
 
 /* ---------------------------------------------------------------------------------
  * GPU: drop-in twins of the per-iteration entry points (device pointers).

@@ -1,7 +1,6 @@
-// Device-side mesh setup, bit-exact twins of the host setup (src/matrix.cc:25-91 and
-// DC_create_nodeToElem as called at src/main.cc:243-250): for meshes that arrive at a
-// server already resident on the GPU. Greedy first-fit coloring
-// (src/coloring.cc:84-109) is a sequential scan over elements and stays on the host.
+// Device-side mesh setup, bit-exact twins of the host setup (src/matrix.cc:25-91,
+// DC_create_nodeToElem as called at src/main.cc:243-250, and the greedy coloring of
+// src/coloring.cc:46-109): for meshes that arrive at a server already resident on the GPU.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -134,7 +133,294 @@ int scan_exclusive(const int *in, int *out, int n, hipStream_t s)
 
 inline dim3 grid_for(int64_t n, int bt) { return dim3((unsigned)((n + bt - 1) / bt)); }
 
+// ---- greedy first-fit coloring (src/coloring.cc:46-81), exact, level by level ----
+//
+// Element e takes the lowest color that no lower-index element sharing a node with it
+// holds. At node v the elements are listed in ascending order (nodeToElem), so e's color
+// depends on v's list only up to e. Once every element before e in each of its nodes'
+// lists is colored, e's color is fixed: the first color missing from the OR of its nodes'
+// color masks, each mask holding the colors of the elements colored at that node so far
+// (the host scan's per-node masks). Elements become ready in levels: level 0 has no lower
+// neighbour at all, and an element is ready once its predecessor at each node is colored.
+// Two elements ready in the same level never share a node (the later one's predecessor
+// at a shared node would be uncolored), so one level colors its elements in parallel, and
+// the result is the sequential scan's, bit for bit. The 100^3 cube needs 2,750 levels of
+// about 2,100 elements each.
+
+// Per element: at each distinct node, the element that follows it in the node's list
+// (-1: none), and the number of distinct nodes where an element precedes it.
+__global__ void k_color_links(const int4 *__restrict__ e2n, int nE, const int *__restrict__ index,
+                              const int *__restrict__ value, int4 *__restrict__ next,
+                              int *__restrict__ npre)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nE) return;
+    const int4 q = e2n[e];
+    const int v[4] = {q.x - 1, q.y - 1, q.z - 1, q.w - 1};
+    int nx[4] = {-1, -1, -1, -1}, np = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        bool dup = false;
+#pragma unroll
+        for (int j = 0; j < k; j++) dup |= v[j] == v[k];
+        if (dup) continue;  // a node the element repeats counts once
+        const int b = index[v[k]], en = index[v[k] + 1];
+        int lo = b, hi = en;  // first entry >= e
+        while (lo < hi) {
+            const int m = (lo + hi) >> 1;
+            if (value[m] < e) lo = m + 1;
+            else hi = m;
+        }
+        int last = lo;  // past e's entries (two when the element repeats the node)
+        while (last < en && value[last] == e) last++;
+        np += lo > b;
+        if (last < en) nx[k] = value[last];
+    }
+    next[e] = make_int4(nx[0], nx[1], nx[2], nx[3]);
+    npre[e] = np;
+}
+
+__global__ void k_color_seed(const int *__restrict__ npre, int nE, int *__restrict__ front,
+                             int *__restrict__ count)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nE && npre[e] == 0) front[atomicAdd(count, 1)] = e;
+}
+
+// All levels in one workgroup: a barrier between levels orders them, with no wait on other
+// workgroups. A level's cost is its chain of memory round trips, so each thread takes up
+// to kColorBatch elements at once and issues every load and atomic of the batch before
+// using any (the elements of one level share no node, so no load depends on another
+// element's stores). The lists of levels after the first are kept in LDS up to
+// kFrontLds entries, the rest in front_a/front_b. status: [0] levels, [1] elements
+// colored, [2] nonzero when an element needs more than MF_MAX_COLOR colors.
+constexpr int kColorThreads = 1024, kColorBatch = 3, kFrontLds = 8192;
+
+__global__ void __launch_bounds__(kColorThreads)
+k_color_levels(const int4 *__restrict__ e2n, int nE, const int4 *__restrict__ next,
+               const int *__restrict__ npre, int *__restrict__ cnt, int *front_a,
+               int *front_b, const int *__restrict__ n0, ulonglong2 *masks,
+               int *__restrict__ part, int *__restrict__ status)
+{
+    __shared__ int s_front[2][kFrontLds];
+    __shared__ int s_tail, s_n, s_err;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        s_n = *n0;
+        s_err = 0;
+    }
+    __syncthreads();
+    int n = s_n, cb = 0, levels = 0;
+    bool from_global = true;  // the first level's list is front_a, written by k_color_seed
+    int *gcur = front_a, *gnxt = front_b;
+    int64_t colored = 0;
+    while (n > 0 && levels <= nE) {
+        if (tid == 0) s_tail = 0;
+        __syncthreads();
+        for (int base = 0; base < n; base += kColorBatch * kColorThreads) {
+            int e[kColorBatch];
+            bool on[kColorBatch];
+#pragma unroll
+            for (int j = 0; j < kColorBatch; j++) {
+                const int i = base + tid + j * kColorThreads;
+                on[j] = i < n;
+                e[j] = !on[j] ? 0 : (from_global || i >= kFrontLds) ? gcur[i] : s_front[cb][i];
+            }
+            int4 q[kColorBatch], nq[kColorBatch];
+#pragma unroll
+            for (int j = 0; j < kColorBatch; j++) {
+                q[j] = on[j] ? e2n[e[j]] : make_int4(1, 1, 1, 1);
+                nq[j] = on[j] ? next[e[j]] : make_int4(-1, -1, -1, -1);
+            }
+            ulonglong2 m[kColorBatch][4];
+            int old[kColorBatch][4], np[kColorBatch][4];
+#pragma unroll
+            for (int j = 0; j < kColorBatch; j++) {
+                const int v[4] = {q[j].x - 1, q[j].y - 1, q[j].z - 1, q[j].w - 1};
+                const int c[4] = {nq[j].x, nq[j].y, nq[j].z, nq[j].w};
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    m[j][k] = on[j] ? masks[v[k]] : make_ulonglong2(0, 0);
+                    old[j][k] = c[k] >= 0 ? atomicAdd(cnt + c[k], 1) : 0;
+                    np[j][k] = c[k] >= 0 ? npre[c[k]] : 0;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kColorBatch; j++) {
+                if (!on[j]) continue;
+                const uint64_t lo = m[j][0].x | m[j][1].x | m[j][2].x | m[j][3].x;
+                const uint64_t hi = m[j][0].y | m[j][1].y | m[j][2].y | m[j][3].y;
+                int color = ~lo ? __builtin_ctzll(~lo) : ~hi ? 64 + __builtin_ctzll(~hi) : 128;
+                if (color >= MF_MAX_COLOR) {
+                    s_err = 1;
+                    color = MF_MAX_COLOR - 1;  // keeps the levels going; the call fails
+                }
+                part[e[j]] = color;
+                const uint64_t blo = color < 64 ? 1ull << color : 0ull;
+                const uint64_t bhi = color < 64 ? 0ull : 1ull << (color - 64);
+                const int v[4] = {q[j].x - 1, q[j].y - 1, q[j].z - 1, q[j].w - 1};
+                // no other element of this level holds these nodes (a repeated node gets
+                // the same value twice)
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    masks[v[k]] = make_ulonglong2(m[j][k].x | blo, m[j][k].y | bhi);
+                const int c[4] = {nq[j].x, nq[j].y, nq[j].z, nq[j].w};
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    // no successor at this node, or c[k] still waits for another node
+                    if (c[k] < 0 || old[j][k] + 1 != np[j][k]) continue;
+                    const int pos = atomicAdd(&s_tail, 1);
+                    if (pos < kFrontLds) s_front[cb ^ 1][pos] = c[k];
+                    else gnxt[pos] = c[k];
+                }
+            }
+        }
+        __syncthreads();  // the level's colors, masks and the next level's list are written
+        colored += n;
+        n = s_tail;
+        cb ^= 1;
+        from_global = false;
+        int *t = gcur;
+        gcur = gnxt;
+        gnxt = t;
+        levels++;
+        __syncthreads();  // everyone has read s_tail before it is reset
+    }
+    if (tid == 0) {
+        status[0] = levels;
+        status[1] = (int)colored;
+        status[2] = s_err;
+    }
+}
+
+__global__ void k_iota(int *__restrict__ a, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+// Elements sorted by (color, index): position i holds element order[i], whose new index
+// is i (the stable counting sort of src/coloring.cc:27-42 and DC_create_permutation).
+// Each color's first position goes to colorToElem.
+__global__ void k_color_perm(const int *__restrict__ key, const int *__restrict__ order, int nE,
+                             int *__restrict__ perm, int *__restrict__ c2e)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nE) return;
+    perm[order[i]] = i;
+    if (i == 0 || key[i] != key[i - 1]) c2e[key[i]] = i;
+    if (i == nE - 1) c2e[key[i] + 1] = nE;
+}
+
+__global__ void k_check_perm(const int *__restrict__ perm, int n, int *__restrict__ bad)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && (perm[i] < 0 || perm[i] >= n)) atomicOr(bad, 1);
+}
+
+// row i of src to row perm[i] of dst
+template <typename T>
+__global__ void k_permute_rows(const T *__restrict__ src, const int *__restrict__ perm, int n,
+                               int dim, T *__restrict__ dst)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * dim) return;
+    const int64_t i = t / dim, d = t - i * dim;
+    dst[(int64_t)perm[i] * dim + d] = src[t];
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { (void)hipFree(p); }
+    template <typename T>
+    T *get() const { return static_cast<T *>(p); }
+};
+
+inline int dev_alloc(DevBuf &b, size_t bytes)
+{
+    return hipMalloc(&b.p, std::max<size_t>(bytes, 16)) == hipSuccess ? MF_OK : MF_ERR_NOMEM;
+}
+
 }  // namespace
+
+extern "C" int mf_dev_coloring_creation(const int *elemToNode, int *colorPart, int *colorPerm,
+                                        int nbElem, int nbNodes, int *colorToElem,
+                                        int *nbColors, void *stream)
+{
+    if (nbElem < 0 || nbNodes < 0 || !colorPerm || !colorToElem || !nbColors ||
+        (nbElem > 0 && !elemToNode) || (reinterpret_cast<uintptr_t>(elemToNode) & 15))
+        return MF_ERR_ARG;  // elements are read as 16-byte rows
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (nbElem == 0) {
+        *nbColors = 0;
+        const int z = 0;
+        if (hipMemcpyAsync(colorToElem, &z, sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return MF_ERR_HIP;
+        return MF_OK;
+    }
+    const size_t nE = (size_t)nbElem, N = (size_t)nbNodes;
+    DevBuf index, value, next, npre, cnt, fa, fb, masks, part, misc, keys, order, tmp;
+    int st = MF_OK;
+    if (st == MF_OK) st = dev_alloc(index, (N + 1) * sizeof(int));
+    if (st == MF_OK) st = dev_alloc(value, 4 * nE * sizeof(int));
+    if (st == MF_OK) st = dev_alloc(next, nE * sizeof(int4));
+    if (st == MF_OK) st = dev_alloc(npre, nE * sizeof(int));
+    if (st == MF_OK) st = dev_alloc(cnt, nE * sizeof(int));
+    if (st == MF_OK) st = dev_alloc(fa, nE * sizeof(int));
+    if (st == MF_OK) st = dev_alloc(fb, nE * sizeof(int));
+    if (st == MF_OK) st = dev_alloc(masks, N * sizeof(ulonglong2));
+    if (st == MF_OK) st = dev_alloc(misc, 4 * sizeof(int));  // seed count, status[3]
+    if (st == MF_OK && !colorPart) st = dev_alloc(part, nE * sizeof(int));
+    if (st != MF_OK) return st;
+    int *partp = colorPart ? colorPart : part.get<int>();
+    // nodeToElem (each node's elements ascending), as the host's
+    st = mf_dev_create_nodeToElem(elemToNode, nbElem, nbNodes, index.get<int>(), value.get<int>(),
+                                  stream);
+    if (st != MF_OK) return st;
+    const int4 *e2n = reinterpret_cast<const int4 *>(elemToNode);
+    int *seed = misc.get<int>(), *status = seed + 1;
+    if (hipMemsetAsync(cnt.p, 0, nE * sizeof(int), s) != hipSuccess ||
+        hipMemsetAsync(masks.p, 0, N * sizeof(ulonglong2), s) != hipSuccess ||
+        hipMemsetAsync(misc.p, 0, 4 * sizeof(int), s) != hipSuccess)
+        return MF_ERR_HIP;
+    hipLaunchKernelGGL(k_color_links, grid_for(nbElem, 256), dim3(256), 0, s, e2n, nbElem,
+                       index.get<int>(), value.get<int>(), next.get<int4>(), npre.get<int>());
+    hipLaunchKernelGGL(k_color_seed, grid_for(nbElem, 256), dim3(256), 0, s, npre.get<int>(),
+                       nbElem, fa.get<int>(), seed);
+    hipLaunchKernelGGL(k_color_levels, dim3(1), dim3(kColorThreads), 0, s, e2n, nbElem,
+                       next.get<int4>(), npre.get<int>(), cnt.get<int>(), fa.get<int>(),
+                       fb.get<int>(), seed, masks.get<ulonglong2>(), partp, status);
+    if (hipGetLastError() != hipSuccess) return MF_ERR_HIP;
+    int hs[3] = {0, 0, 0};
+    if (hipMemcpyAsync(hs, status, sizeof(hs), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return MF_ERR_HIP;
+    if (hs[2]) return MF_ERR_COLORS;
+    if (hs[1] != nbElem) return MF_ERR_ARG;  // not every element became ready
+    // stable sort of the element ids by color: 7 key bits (colors < 128)
+    if (dev_alloc(keys, nE * sizeof(int)) != MF_OK || dev_alloc(order, 2 * nE * sizeof(int)) != MF_OK)
+        return MF_ERR_NOMEM;
+    int *kout = keys.get<int>(), *vin = order.get<int>(), *vout = vin + nE;
+    hipLaunchKernelGGL(k_iota, grid_for(nbElem, 256), dim3(256), 0, s, vin, nbElem);
+    size_t tb = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, tb, partp, kout, vin, vout, nbElem, 0, 7, s) !=
+        hipSuccess)
+        return MF_ERR_HIP;
+    if (dev_alloc(tmp, tb) != MF_OK) return MF_ERR_NOMEM;
+    if (hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, partp, kout, vin, vout, nbElem, 0, 7, s) !=
+        hipSuccess)
+        return MF_ERR_HIP;
+    hipLaunchKernelGGL(k_color_perm, grid_for(nbElem, 256), dim3(256), 0, s, kout, vout, nbElem,
+                       colorPerm, colorToElem);
+    if (hipGetLastError() != hipSuccess) return MF_ERR_HIP;
+    int last = 0;
+    if (hipMemcpyAsync(&last, kout + nE - 1, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return MF_ERR_HIP;
+    *nbColors = last + 1;
+    return MF_OK;
+}
 
 extern "C" int mf_dev_create_nodeToElem(const int *elemToNode, int nbElem, int nbNodes, int *index,
                                         int *value, void *stream)
@@ -224,4 +510,43 @@ extern "C" int mf_dev_create_elemToEdge(const int *row, const int *col, const in
                        col, elemToNode, nbElem, elemToEdge);
     if (hipGetLastError() != hipSuccess) return MF_ERR_HIP;
     return hipStreamSynchronize(s) == hipSuccess ? MF_OK : MF_ERR_HIP;
+}
+
+namespace {
+
+template <typename T>
+int dev_permute_rows(T *tab, const int *perm, int n, int dim, hipStream_t s)
+{
+    if (n < 0 || dim <= 0 || (n > 0 && (!tab || !perm))) return MF_ERR_ARG;
+    if (n == 0) return MF_OK;
+    DevBuf tmp, bad;
+    if (dev_alloc(tmp, (size_t)n * dim * sizeof(T)) != MF_OK || dev_alloc(bad, sizeof(int)) != MF_OK)
+        return MF_ERR_NOMEM;
+    if (hipMemsetAsync(bad.p, 0, sizeof(int), s) != hipSuccess) return MF_ERR_HIP;
+    hipLaunchKernelGGL(k_check_perm, grid_for(n, 256), dim3(256), 0, s, perm, n, bad.get<int>());
+    int hbad = 0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return MF_ERR_HIP;
+    if (hbad) return MF_ERR_ARG;  // tab unchanged, as the host twin leaves it
+    const int64_t total = (int64_t)n * dim;
+    if (hipMemcpyAsync(tmp.p, tab, (size_t)total * sizeof(T), hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return MF_ERR_HIP;
+    hipLaunchKernelGGL(k_permute_rows<T>, grid_for(total, 256), dim3(256), 0, s, tmp.get<T>(), perm,
+                       n, dim, tab);
+    if (hipGetLastError() != hipSuccess) return MF_ERR_HIP;
+    return hipStreamSynchronize(s) == hipSuccess ? MF_OK : MF_ERR_HIP;
+}
+
+}  // namespace
+
+extern "C" int mf_dev_permute_int_2d(int *tab, const int *perm, int n, int dim, void *stream)
+{
+    return dev_permute_rows(tab, perm, n, dim, reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int mf_dev_permute_double_2d(double *tab, const int *perm, int n, int dim, void *stream)
+{
+    return dev_permute_rows(tab, perm, n, dim, reinterpret_cast<hipStream_t>(stream));
 }

@@ -78,6 +78,9 @@ SIGNATURES = {
     "mf_dev_create_nodeToElem": (_i, [_p, _i, _i, _p, _p, _p]),
     "mf_dev_create_nodeToNode": (_i, [_p, _p, _p, _p, _p, _i, _i, _p]),
     "mf_dev_create_elemToEdge": (_i, [_p, _p, _p, _p, _i, _p]),
+    "mf_dev_coloring_creation": (_i, [_p, _p, _p, _i, _i, _p, _p, _p]),
+    "mf_dev_permute_int_2d": (_i, [_p, _p, _i, _i, _p]),
+    "mf_dev_permute_double_2d": (_i, [_p, _p, _i, _i, _p]),
     "mf_pcg": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p, ctypes.c_double, _i, _p, _p, _p]),
     "mf_host_alloc": (_i, [_p, ctypes.c_size_t]),
     "mf_host_free": (_i, [_p]),

@@ -255,10 +255,10 @@ def norm(x, stream=None):
     return float(out[0])
 
 
-def device_setup(elemToNode, nbNodes, nbEdges, stream=None):
+def device_setup(elemToNode, nbNodes, nbEdges, stream=None, with_elem_to_edge=True):
     """nodeToElem, the CSR pattern and elemToEdge built on the GPU from a (color-permuted,
     1-based) elemToNode: bit-exact twins of the host setup. Returns DeviceArrays
-    (index, value, row, col, elemToEdge)."""
+    (index, value, row, col, elemToEdge or None) and the nnz."""
     _ensure()
     e2n = elemToNode if isinstance(elemToNode, (DeviceArray, DeviceView)) else \
         DeviceArray.from_numpy(np.asarray(elemToNode, np.int32))
@@ -272,10 +272,37 @@ def device_setup(elemToNode, nbNodes, nbEdges, stream=None):
     nnz = check(lib().mf_dev_create_nodeToNode(row.ptr, col.ptr, index.ptr, value.ptr, e2n.ptr,
                                                int(nbNodes), int(nbEdges), _sp(stream)),
                 "mf_dev_create_nodeToNode")
+    if not with_elem_to_edge:
+        return index, value, row, col, None, nnz
     e2e = DeviceArray(max(16 * nE, 1), np.int32)
     check(lib().mf_dev_create_elemToEdge(row.ptr, col.ptr, e2n.ptr, e2e.ptr, nE, _sp(stream)),
           "mf_dev_create_elemToEdge")
     return index, value, row, col, e2e, nnz
+
+
+def coloring_creation(elemToNode, nbNodes, stream=None):
+    """Greedy first-fit coloring on the GPU (mf_dev_coloring_creation), equal to the host
+    scan: (colorPart, colorPerm) as DeviceArrays, colorToElem as a numpy array of
+    nbColors + 1 offsets."""
+    _ensure()
+    e2n = elemToNode if isinstance(elemToNode, (DeviceArray, DeviceView)) else \
+        DeviceArray.from_numpy(np.asarray(elemToNode, np.int32))
+    nE = e2n.size // 4
+    part = DeviceArray(max(nE, 1), np.int32)
+    perm = DeviceArray(max(nE, 1), np.int32)
+    c2e = DeviceArray(129, np.int32)
+    nc = ctypes.c_int()
+    check(lib().mf_dev_coloring_creation(e2n.ptr, part.ptr, perm.ptr, nE, int(nbNodes), c2e.ptr,
+                                         ctypes.byref(nc), _sp(stream)), "mf_dev_coloring_creation")
+    return part, perm, c2e.numpy()[:nc.value + 1].copy()
+
+
+def permute_2d(tab, perm, dim, stream=None):
+    """Row i of the device array tab moves to row perm[i] (mf_dev_permute_*_2d)."""
+    n = tab.size // dim
+    fn = lib().mf_dev_permute_int_2d if np.dtype(tab.dtype) == np.int32 else \
+        lib().mf_dev_permute_double_2d
+    check(fn(tab.ptr, perm.ptr, n, int(dim), _sp(stream)), "mf_dev_permute_2d")
 
 
 def pcg(operator_id, nbNodes, row, col, values, prec, checkBounds, b, x, rtol=1e-10,

@@ -165,10 +165,16 @@ class Mesh:
     Attributes: nbElem, nbNodes, nnz, coord, elemToNode (permuted), coef (permuted or
     None), colorPerm, colorToElem, row, col, checkBounds, operatorID, dim;
     elemToNode_original / coef_original keep the input element order.
+
+    setup="device" runs the coloring, the permutations and the CSR pattern on the GPU
+    (mf_dev_coloring_creation, mf_dev_permute_*, mf_dev_create_*) and copies the results
+    back; they equal the host setup's bit for bit.
     """
 
     def __init__(self, coord, elemToNode, boundNodesCode, nbEdges, operator_id, coef=None,
-                 colored=True):
+                 colored=True, setup="host"):
+        if setup not in ("host", "device"):
+            raise ValueError(f"setup must be 'host' or 'device', got {setup!r}")
         self.operatorID = int(operator_id)
         self.dim = 9 if self.operatorID == OP_ELA else 1
         self.coord = _c(coord, np.float64)
@@ -177,7 +183,9 @@ class Mesh:
         self.coef_original = None if coef is None else _c(coef, np.float64)
         self.nbElem = e2n.size // 4
         self.nbNodes = self.coord.size // 3
-        if colored:
+        if colored and setup == "device":
+            self._device_setup(e2n, coef, nbEdges)
+        elif colored:
             _, self.colorPerm, self.colorToElem = coloring_creation(e2n, self.nbNodes)
             e2n = permute_2d(e2n, self.colorPerm, 4)
             if coef is not None:
@@ -185,12 +193,31 @@ class Mesh:
         else:
             self.colorPerm = np.arange(self.nbElem, dtype=np.int32)
             self.colorToElem = np.array([0, self.nbElem], np.int32)
-        self.elemToNode = e2n
-        self.coef = None if coef is None else _c(coef, np.float64)
-        self.row, self.col = create_nodeToNode(e2n, self.nbNodes, nbEdges)
+        if not (colored and setup == "device"):
+            self.elemToNode = e2n
+            self.coef = None if coef is None else _c(coef, np.float64)
+            self.row, self.col = create_nodeToNode(e2n, self.nbNodes, nbEdges)
         self.nnz = int(self.row[-1])
         self.boundNodesCode = _c(boundNodesCode, np.int32)
         self.checkBounds = boundary_mask(self.boundNodesCode)
+
+    def _device_setup(self, e2n, coef, nbEdges):
+        from . import gpu
+        d_e2n = gpu.DeviceArray.from_numpy(e2n)
+        _, d_perm, self.colorToElem = gpu.coloring_creation(d_e2n, self.nbNodes)
+        gpu.permute_2d(d_e2n, d_perm, 4)
+        self.coef = None
+        if coef is not None:
+            c = _c(coef, np.float64)
+            d_coef = gpu.DeviceArray.from_numpy(c)
+            gpu.permute_2d(d_coef, d_perm, c.size // self.nbElem)
+            self.coef = d_coef.numpy()
+        _, _, row, col, _, nnz = gpu.device_setup(d_e2n, self.nbNodes, nbEdges,
+                                                  with_elem_to_edge=False)
+        self.colorPerm = d_perm.numpy()[:self.nbElem].copy()
+        self.elemToNode = d_e2n.numpy()
+        self.row = row.numpy()
+        self.col = col.numpy()[:nnz].copy()
 
     @property
     def nbColors(self):
