@@ -145,7 +145,7 @@ inline dim3 grid_for(int64_t n, int bt) { return dim3((unsigned)((n + bt - 1) / 
 // Two elements ready in the same level never share a node (the later one's predecessor
 // at a shared node would be uncolored), so one level colors its elements in parallel, and
 // the result is the sequential scan's, bit for bit. The 100^3 cube needs 2,750 levels of
-// about 2,100 elements each.
+// about 2,100 elements each, one launch per level.
 
 // Per element: at each distinct node, the element that follows it in the node's list
 // (-1: none), and the number of distinct nodes where an element precedes it.
@@ -187,109 +187,60 @@ __global__ void k_color_seed(const int *__restrict__ npre, int nE, int *__restri
     if (e < nE && npre[e] == 0) front[atomicAdd(count, 1)] = e;
 }
 
-// All levels in one workgroup: a barrier between levels orders them, with no wait on other
-// workgroups. A level's cost is its chain of memory round trips, so each thread takes up
-// to kColorBatch elements at once and issues every load and atomic of the batch before
-// using any (the elements of one level share no node, so no load depends on another
-// element's stores). The lists of levels after the first are kept in LDS up to
-// kFrontLds entries, the rest in front_a/front_b. status: [0] levels, [1] elements
-// colored, [2] nonzero when an element needs more than MF_MAX_COLOR colors.
-constexpr int kColorThreads = 1024, kColorBatch = 3, kFrontLds = 8192;
+// One level per launch: launches on one stream run in order, which orders the levels, and
+// each level spreads over kLevelBlocks workgroups (a level's cost is its chain of memory
+// round trips, so more CUs keep more of them in flight). Level L reads its list from
+// front[L & 1] and its length from counts[L % 3], appends the next level to
+// front[(L + 1) & 1] and counts[(L + 1) % 3], and clears counts[(L + 2) % 3] for level L + 1.
+// status: [1] elements colored, [2] nonzero when an element needs more than MF_MAX_COLOR
+// colors.
+constexpr int kLevelThreads = 256, kLevelBlocks = 32, kLevelsPerCheck = 64;
 
-__global__ void __launch_bounds__(kColorThreads)
-k_color_levels(const int4 *__restrict__ e2n, int nE, const int4 *__restrict__ next,
-               const int *__restrict__ npre, int *__restrict__ cnt, int *front_a,
-               int *front_b, const int *__restrict__ n0, ulonglong2 *masks,
-               int *__restrict__ part, int *__restrict__ status)
+__global__ void __launch_bounds__(kLevelThreads)
+k_color_level(const int4 *__restrict__ e2n, const int4 *__restrict__ next,
+              const int *__restrict__ npre, int *__restrict__ cnt, const int *__restrict__ cur,
+              int *__restrict__ nxt, int *__restrict__ counts, int L, ulonglong2 *masks,
+              int *__restrict__ part, int *__restrict__ status)
 {
-    __shared__ int s_front[2][kFrontLds];
-    __shared__ int s_tail, s_n, s_err;
-    const int tid = threadIdx.x;
-    if (tid == 0) {
-        s_n = *n0;
-        s_err = 0;
+    const int n = counts[L % 3];
+    const int gt = blockIdx.x * kLevelThreads + threadIdx.x;
+    if (gt == 0) {
+        counts[(L + 2) % 3] = 0;
+        if (n > 0) atomicAdd(status + 1, n);
     }
-    __syncthreads();
-    int n = s_n, cb = 0, levels = 0;
-    bool from_global = true;  // the first level's list is front_a, written by k_color_seed
-    int *gcur = front_a, *gnxt = front_b;
-    int64_t colored = 0;
-    while (n > 0 && levels <= nE) {
-        if (tid == 0) s_tail = 0;
-        __syncthreads();
-        for (int base = 0; base < n; base += kColorBatch * kColorThreads) {
-            int e[kColorBatch];
-            bool on[kColorBatch];
+    for (int i = gt; i < n; i += kLevelBlocks * kLevelThreads) {
+        const int e = cur[i];
+        const int4 q = e2n[e], nq = next[e];
+        const int v[4] = {q.x - 1, q.y - 1, q.z - 1, q.w - 1};
+        const int c[4] = {nq.x, nq.y, nq.z, nq.w};
+        ulonglong2 m[4];
+        int old[4], np[4];
 #pragma unroll
-            for (int j = 0; j < kColorBatch; j++) {
-                const int i = base + tid + j * kColorThreads;
-                on[j] = i < n;
-                e[j] = !on[j] ? 0 : (from_global || i >= kFrontLds) ? gcur[i] : s_front[cb][i];
-            }
-            int4 q[kColorBatch], nq[kColorBatch];
-#pragma unroll
-            for (int j = 0; j < kColorBatch; j++) {
-                q[j] = on[j] ? e2n[e[j]] : make_int4(1, 1, 1, 1);
-                nq[j] = on[j] ? next[e[j]] : make_int4(-1, -1, -1, -1);
-            }
-            ulonglong2 m[kColorBatch][4];
-            int old[kColorBatch][4], np[kColorBatch][4];
-#pragma unroll
-            for (int j = 0; j < kColorBatch; j++) {
-                const int v[4] = {q[j].x - 1, q[j].y - 1, q[j].z - 1, q[j].w - 1};
-                const int c[4] = {nq[j].x, nq[j].y, nq[j].z, nq[j].w};
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    m[j][k] = on[j] ? masks[v[k]] : make_ulonglong2(0, 0);
-                    old[j][k] = c[k] >= 0 ? atomicAdd(cnt + c[k], 1) : 0;
-                    np[j][k] = c[k] >= 0 ? npre[c[k]] : 0;
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < kColorBatch; j++) {
-                if (!on[j]) continue;
-                const uint64_t lo = m[j][0].x | m[j][1].x | m[j][2].x | m[j][3].x;
-                const uint64_t hi = m[j][0].y | m[j][1].y | m[j][2].y | m[j][3].y;
-                int color = ~lo ? __builtin_ctzll(~lo) : ~hi ? 64 + __builtin_ctzll(~hi) : 128;
-                if (color >= MF_MAX_COLOR) {
-                    s_err = 1;
-                    color = MF_MAX_COLOR - 1;  // keeps the levels going; the call fails
-                }
-                part[e[j]] = color;
-                const uint64_t blo = color < 64 ? 1ull << color : 0ull;
-                const uint64_t bhi = color < 64 ? 0ull : 1ull << (color - 64);
-                const int v[4] = {q[j].x - 1, q[j].y - 1, q[j].z - 1, q[j].w - 1};
-                // no other element of this level holds these nodes (a repeated node gets
-                // the same value twice)
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-                    masks[v[k]] = make_ulonglong2(m[j][k].x | blo, m[j][k].y | bhi);
-                const int c[4] = {nq[j].x, nq[j].y, nq[j].z, nq[j].w};
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    // no successor at this node, or c[k] still waits for another node
-                    if (c[k] < 0 || old[j][k] + 1 != np[j][k]) continue;
-                    const int pos = atomicAdd(&s_tail, 1);
-                    if (pos < kFrontLds) s_front[cb ^ 1][pos] = c[k];
-                    else gnxt[pos] = c[k];
-                }
-            }
+        for (int k = 0; k < 4; k++) {
+            m[k] = masks[v[k]];
+            old[k] = c[k] >= 0 ? atomicAdd(cnt + c[k], 1) : 0;
+            np[k] = c[k] >= 0 ? npre[c[k]] : 0;
         }
-        __syncthreads();  // the level's colors, masks and the next level's list are written
-        colored += n;
-        n = s_tail;
-        cb ^= 1;
-        from_global = false;
-        int *t = gcur;
-        gcur = gnxt;
-        gnxt = t;
-        levels++;
-        __syncthreads();  // everyone has read s_tail before it is reset
-    }
-    if (tid == 0) {
-        status[0] = levels;
-        status[1] = (int)colored;
-        status[2] = s_err;
+        const uint64_t lo = m[0].x | m[1].x | m[2].x | m[3].x;
+        const uint64_t hi = m[0].y | m[1].y | m[2].y | m[3].y;
+        int color = ~lo ? __builtin_ctzll(~lo) : ~hi ? 64 + __builtin_ctzll(~hi) : 128;
+        if (color >= MF_MAX_COLOR) {
+            atomicOr(status + 2, 1);
+            color = MF_MAX_COLOR - 1;  // keeps the levels going; the call fails
+        }
+        part[e] = color;
+        const uint64_t blo = color < 64 ? 1ull << color : 0ull;
+        const uint64_t bhi = color < 64 ? 0ull : 1ull << (color - 64);
+        // no other element of this level holds these nodes (a repeated node gets the same
+        // value twice)
+#pragma unroll
+        for (int k = 0; k < 4; k++) masks[v[k]] = make_ulonglong2(m[k].x | blo, m[k].y | bhi);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            // no successor at this node, or c[k] still waits for another node
+            if (c[k] < 0 || old[k] + 1 != np[k]) continue;
+            nxt[atomicAdd(counts + (L + 1) % 3, 1)] = c[k];
+        }
     }
 }
 
@@ -370,7 +321,7 @@ extern "C" int mf_dev_coloring_creation(const int *elemToNode, int *colorPart, i
     if (st == MF_OK) st = dev_alloc(fa, nE * sizeof(int));
     if (st == MF_OK) st = dev_alloc(fb, nE * sizeof(int));
     if (st == MF_OK) st = dev_alloc(masks, N * sizeof(ulonglong2));
-    if (st == MF_OK) st = dev_alloc(misc, 4 * sizeof(int));  // seed count, status[3]
+    if (st == MF_OK) st = dev_alloc(misc, 6 * sizeof(int));  // counts[3], status[3]
     if (st == MF_OK && !colorPart) st = dev_alloc(part, nE * sizeof(int));
     if (st != MF_OK) return st;
     int *partp = colorPart ? colorPart : part.get<int>();
@@ -379,19 +330,33 @@ extern "C" int mf_dev_coloring_creation(const int *elemToNode, int *colorPart, i
                                   stream);
     if (st != MF_OK) return st;
     const int4 *e2n = reinterpret_cast<const int4 *>(elemToNode);
-    int *seed = misc.get<int>(), *status = seed + 1;
+    int *counts = misc.get<int>(), *status = counts + 3;
     if (hipMemsetAsync(cnt.p, 0, nE * sizeof(int), s) != hipSuccess ||
         hipMemsetAsync(masks.p, 0, N * sizeof(ulonglong2), s) != hipSuccess ||
-        hipMemsetAsync(misc.p, 0, 4 * sizeof(int), s) != hipSuccess)
+        hipMemsetAsync(misc.p, 0, 6 * sizeof(int), s) != hipSuccess)
         return MF_ERR_HIP;
     hipLaunchKernelGGL(k_color_links, grid_for(nbElem, 256), dim3(256), 0, s, e2n, nbElem,
                        index.get<int>(), value.get<int>(), next.get<int4>(), npre.get<int>());
     hipLaunchKernelGGL(k_color_seed, grid_for(nbElem, 256), dim3(256), 0, s, npre.get<int>(),
-                       nbElem, fa.get<int>(), seed);
-    hipLaunchKernelGGL(k_color_levels, dim3(1), dim3(kColorThreads), 0, s, e2n, nbElem,
-                       next.get<int4>(), npre.get<int>(), cnt.get<int>(), fa.get<int>(),
-                       fb.get<int>(), seed, masks.get<ulonglong2>(), partp, status);
+                       nbElem, fa.get<int>(), counts);
     if (hipGetLastError() != hipSuccess) return MF_ERR_HIP;
+    // levels in batches of kLevelsPerCheck launches, then a look at the next level's
+    // length: launches past the last level find an empty list and return at once
+    int *front[2] = {fa.get<int>(), fb.get<int>()};
+    int L = 0, remaining = 1;
+    while (remaining > 0) {
+        if (L > nbElem) return MF_ERR_ARG;  // more levels than elements cannot happen
+        for (int b = 0; b < kLevelsPerCheck; b++, L++)
+            hipLaunchKernelGGL(k_color_level, dim3(kLevelBlocks), dim3(kLevelThreads), 0, s, e2n,
+                               next.get<int4>(), npre.get<int>(), cnt.get<int>(), front[L & 1],
+                               front[(L + 1) & 1], counts, L, masks.get<ulonglong2>(), partp,
+                               status);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(&remaining, counts + L % 3, sizeof(int), hipMemcpyDeviceToHost, s) !=
+                hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return MF_ERR_HIP;
+    }
     int hs[3] = {0, 0, 0};
     if (hipMemcpyAsync(hs, status, sizeof(hs), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)

@@ -42,9 +42,8 @@ def _host_coloring(mf, e2n, N):
 def test_device_coloring_equals_host(mf, gpu, case):
     """mf_dev_coloring_creation against the host scan (which the oracle pins): colors,
     permutation and color offsets equal, on ordered and shuffled cubes, cubes whose
-    elements come in random order (few, wide levels: the 60^3 one has levels of more than
-    8,192 elements, beyond the kernel's LDS lists), and random tets on few nodes (valence
-    up to ~120, repeated vertices)."""
+    elements come in random order (few, wide levels: the 60^3 one has levels of ~10^4
+    elements), and random tets on few nodes (valence up to ~120, repeated vertices)."""
     import time
     print(f"[dev coloring] {case}", flush=True)
     rng = np.random.default_rng(11)
