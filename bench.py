@@ -90,6 +90,9 @@ def main():
     ap.add_argument("--copy", type=int, default=0,
                     help="instead of the FEM step, time the plain HBM copy kernel on this many "
                          "bytes (bandwidth reference and byte-counter calibration)")
+    ap.add_argument("--coloring", action="store_true",
+                    help="instead of the FEM step, time the greedy coloring of the --n cube "
+                         "(node ids shuffled) on the GPU against the host scan, same result")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported by this "
@@ -144,6 +147,29 @@ def main():
         print(json.dumps({"gathers": n, "ms": ms, "index_bytes": 8 * n, "dst_bytes": 8 * n,
                           "distinct_lines": lines,
                           "gather_bytes_if_line": {w: lines[w] * w for w in lines}}), flush=True)
+        return
+
+    if args.coloring:
+        gpu.init(0)
+        coord, e2n, _, _ = minifem.generate_cube(args.n, 0.25, 3, 203)
+        N = coord.size // 3
+        d_e2n = gpu.DeviceArray.from_numpy(np.ascontiguousarray(e2n, np.int32))
+        t_dev, t_host = [], []
+        for _ in range(max(args.steps, 1)):
+            gpu.synchronize()
+            t = time.perf_counter()
+            part, perm, c2e = gpu.coloring_creation(d_e2n, N)
+            t_dev.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        hp, hperm, hc2e = minifem.coloring_creation(e2n, N)
+        t_host.append(time.perf_counter() - t)
+        nE = e2n.size // 4
+        same = bool(np.array_equal(part.numpy()[:nE], hp) and np.array_equal(perm.numpy()[:nE], hperm)
+                    and np.array_equal(c2e, hc2e))
+        print(json.dumps({"coloring": f"cube {args.n}^3 node-shuffled, {nE} tets, {len(hc2e) - 1} colors",
+                          "device_ms_median": round(float(np.median(t_dev)) * 1e3, 2),
+                          "device_ms_first": round(t_dev[0] * 1e3, 2),
+                          "host_ms": round(t_host[0] * 1e3, 2), "equal": same}), flush=True)
         return
 
     if args.batched:

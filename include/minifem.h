@@ -360,6 +360,71 @@ int mf_plans_assemble(const mf_plan_job *jobs, int njobs, int precision, void *s
  * have no tiles), or a negative status (MF_ERR_ARG also when they do not fit one CU). */
 int mf_plans_lds_bytes(const mf_plan_job *jobs, int njobs, int precision);
 
+/* ---------------------------------------------------------------------------------
+ * Served form (no counterpart in the reference): a native request loop over resident
+ * meshes. Requests arrive over time (each a fresh coefficient field or fresh coordinates
+ * for one mesh); each is admitted once its input and outputs fit the HBM arena the caller
+ * hands over, and a free stream serves up to max_group admitted requests of one
+ * precision mode that arrived together with one mf_plans_assemble launch. A request that
+ * does not fit waits; later ones may pass it (first fit) until it has been passed
+ * max_bypass times.
+ * ------------------------------------------------------------------------------- */
+
+typedef struct mf_server mf_server;
+
+#define MF_REQ_COEF 0   /* the payload is the mesh's coefficient field */
+#define MF_REQ_COORD 1  /* the payload is the mesh's coordinates */
+
+/* A resident mesh: its plan and device inputs (used where a request does not replace
+ * them), and its sizes: a request's payload must be exactly ncoord doubles
+ * (coordinates), ncoef doubles (a coefficient field) or ncoef floats (a coefficient field
+ * with MF_PREC_MIXED_F32COEF). */
+typedef struct mf_server_mesh {
+    const mf_plan *plan;
+    const double *coord;   /* device, 3N */
+    const double *coef;    /* device, NULL = reference constants */
+    int64_t nvalues;       /* nnz * d */
+    int64_t nprec;         /* N * d */
+    int64_t ncoord;        /* 3N */
+    int64_t ncoef;         /* nE (lap) or 2 nE (ela) */
+} mf_server_mesh;
+
+typedef struct mf_request {
+    int mesh;              /* index of a resident mesh */
+    int kind;              /* MF_REQ_COEF or MF_REQ_COORD */
+    int mode;              /* precision of its launch, MF_PREC_*; MF_PREC_MIXED_F32COEF only
+                              for a float coefficient field */
+    int pad;
+    const void *payload;   /* host memory (pinned, so its upload overlaps other work) */
+    int64_t payload_bytes;
+    double t_arrival;      /* seconds after the start of the run */
+    double *values_out;    /* host, nvalues doubles, or NULL: copied back once served */
+    double *prec_out;      /* host, nprec doubles, or NULL */
+} mf_request;
+
+typedef struct mf_server_stats {
+    int64_t launches;
+    int64_t memory_waits;  /* requests that found a stream free but not their HBM */
+    double makespan_s;     /* first arrival to last completion */
+} mf_server_stats;
+
+/* arena: device memory of arena_bytes that the server carves for request inputs and
+ * outputs (no allocation while serving); slots: streams (1..16); max_group:
+ * 1..MF_MAX_PLAN_JOBS; max_bypass >= 0. */
+int mf_server_create(mf_server **server, void *arena, int64_t arena_bytes, int slots,
+                     int max_group, int max_bypass);
+/* Adds a resident mesh, also between runs; returns its index (>= 0) or a status. */
+int mf_server_add_mesh(mf_server *server, const mf_server_mesh *mesh);
+/* Serves n requests (any order of t_arrival) and returns when all are done: t_start[i]
+ * (admission) and t_done[i] (the end of its launch and copies, as the loop saw it), in
+ * seconds after the start. MF_ERR_BUDGET, before anything runs, when a request needs
+ * more than the arena; MF_ERR_ARG for a bad mesh, kind, mode or payload size. */
+int mf_server_run(mf_server *server, const mf_request *reqs, int n, double *t_start,
+                  double *t_done, mf_server_stats *stats);
+/* Arena bytes held by requests (0 between runs). */
+int64_t mf_server_arena_used(const mf_server *server);
+int mf_server_destroy(mf_server *server);
+
 /* Measurement only, no counterpart in the reference: writes zeros over values[nnz*d]
  * with the assembly's output stream alone, to time what HBM accepts for it. mode 0:
  * every tile row's CSR segment in the order mf_plan_assemble writes them; mode 1: the

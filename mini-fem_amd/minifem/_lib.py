@@ -100,15 +100,39 @@ SIGNATURES = {
     "mf_plan_host_stats": (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p]),
     "mf_plans_assemble": (_i, [_p, _i, _i, _p]),
     "mf_plans_lds_bytes": (_i, [_p, _i, _i]),
+    "mf_server_create": (_i, [_p, _p, _i64, _i, _i, _i]),
+    "mf_server_add_mesh": (_i, [_p, _p]),
+    "mf_server_run": (_i, [_p, _p, _i, _p, _p, _p]),
+    "mf_server_arena_used": (_i64, [_p]),
+    "mf_server_destroy": (_i, [_p]),
 }
 
 MAX_PLAN_JOBS = 16
+REQ_COEF = 0
+REQ_COORD = 1
 
 
 class PlanJob(ctypes.Structure):
     """mf_plan_job: one plan's iteration inside a multi-plan launch."""
     _fields_ = [("plan", _p), ("coord", _p), ("coef", _p), ("values", _p), ("prec", _p),
                 ("singular", _p)]
+
+
+class ServerMesh(ctypes.Structure):
+    """mf_server_mesh: a resident mesh of the native request loop."""
+    _fields_ = [("plan", _p), ("coord", _p), ("coef", _p), ("nvalues", _i64), ("nprec", _i64),
+                ("ncoord", _i64), ("ncoef", _i64)]
+
+
+class Request(ctypes.Structure):
+    """mf_request: one served request."""
+    _fields_ = [("mesh", ctypes.c_int), ("kind", ctypes.c_int), ("mode", ctypes.c_int),
+                ("pad", ctypes.c_int), ("payload", _p), ("payload_bytes", _i64),
+                ("t_arrival", _d), ("values_out", _p), ("prec_out", _p)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("launches", _i64), ("memory_waits", _i64), ("makespan_s", _d)]
 
 
 class PlanParams(ctypes.Structure):

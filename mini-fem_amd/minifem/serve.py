@@ -27,59 +27,25 @@ lower throughput (a merged launch runs every member at the largest member's LDS 
 and lasts as long as its longest job). Uploading on separate copy streams, with the
 compute stream waiting for them on the device or on the host, was slower in every form.
 
-The loop polls completion events and frees memory and streams as launches finish, so new
-arrivals are admitted while others still run. Latency is measured from arrival to
-completion of the launch that served the request.
+The request loop is native (mf_server_run, csrc/host_server.cpp): it polls the streams'
+completion events, frees memory and streams as launches finish, and admits new arrivals
+while others still run. Latency is measured from arrival to the completion of the launch
+that served the request, as the loop saw it.
 
 The mixed-precision server takes a request's coefficient field as float32, the precision
 its element products use (coordinates stay float64: their differences form the element
 Jacobian), which halves those uploads.
 """
-import bisect
-import time
+import ctypes
 
 import numpy as np
 
 from . import gpu
-from ._lib import MAX_PLAN_JOBS, OP_ELA, PREC_FP64, PREC_MIXED, PREC_MIXED_F32COEF
+from ._lib import (MAX_PLAN_JOBS, MF_ERR_BUDGET, OP_ELA, PREC_FP64, PREC_MIXED,
+                   PREC_MIXED_F32COEF, REQ_COEF, REQ_COORD, ServerMesh, ServerStats, check, lib)
+from ._lib import Request as RequestDesc
 from .setup import Mesh, generate_coef, generate_cube
 
-_ALIGN = 256
-
-
-class Arena:
-    """First-fit allocator over [0, size) bytes, 256-byte aligned blocks."""
-
-    def __init__(self, size):
-        self.size = int(size)
-        self._free = [(0, self.size)]  # sorted (offset, length)
-        self.used = 0
-
-    def alloc(self, nbytes):
-        n = (int(nbytes) + _ALIGN - 1) // _ALIGN * _ALIGN
-        for i, (off, ln) in enumerate(self._free):
-            if ln >= n:
-                if ln == n:
-                    self._free.pop(i)
-                else:
-                    self._free[i] = (off + n, ln - n)
-                self.used += n
-                return off, n
-        return None
-
-    def free(self, block):
-        off, n = block
-        i = bisect.bisect_left(self._free, (off, 0))
-        self._free.insert(i, (off, n))
-        # merge with the neighbours
-        if i + 1 < len(self._free) and off + n == self._free[i + 1][0]:
-            self._free[i] = (off, n + self._free[i + 1][1])
-            self._free.pop(i + 1)
-        if i > 0 and self._free[i - 1][0] + self._free[i - 1][1] == off:
-            po, pn = self._free[i - 1]
-            self._free[i - 1] = (po, pn + self._free[i][1])
-            self._free.pop(i)
-        self.used -= n
 
 
 class Request:
@@ -144,8 +110,9 @@ def retime(requests, rate_hz, seed=5000):
 
 
 class Server:
-    """Resident meshes + an HBM budget + `slots` streams; each launch serves up to
-    `max_group` requests at once (slots=1, max_group=1: the plain one-at-a-time server)."""
+    """Resident meshes + an HBM budget + `slots` streams, served by the native request loop
+    (mf_server_run); each launch serves up to `max_group` requests at once (slots=1,
+    max_group=1: the plain one-at-a-time server)."""
 
     def __init__(self, meshes, hbm_budget, slots=4, precision=PREC_FP64, plan_params=None,
                  max_group=2, max_bypass=8):
@@ -156,49 +123,57 @@ class Server:
         self.meshes = list(meshes)
         self.precision = precision
         self.max_group = max(1, min(int(max_group), MAX_PLAN_JOBS))
-        self.max_bypass = max(0, int(max_bypass))
+        self._lib = lib()
+        self._srv = None
         self.dms = [gpu.DeviceMesh(m) for m in self.meshes]
         self._plan_params = dict(plan_params or {})
         self.plans = [gpu.Plan(m, **self._plan_params) for m in self.meshes]
-        self.arena_mem = gpu.DeviceArray(int(hbm_budget), np.uint8)
-        self.arena = Arena(hbm_budget)
-        self.slots = [gpu.Stream() for _ in range(max(1, int(slots)))]
-        self.events = [gpu.Event() for _ in self.slots]
-        self._fits = {}  # (precision, sorted plan indices) -> one launch fits one CU's LDS
+        self.budget = int(hbm_budget)
+        self.arena_mem = gpu.DeviceArray(self.budget, np.uint8)
+        p = ctypes.c_void_p()
+        check(self._lib.mf_server_create(ctypes.byref(p), self.arena_mem.ptr, self.budget,
+                                         int(slots), self.max_group, max(0, int(max_bypass))),
+              "mf_server_create")
+        self._srv = p.value
+        for m, dm, plan in zip(self.meshes, self.dms, self.plans):
+            self._register(m, dm, plan)
         # one warm launch per plan before any timing (code objects loaded, caches warm)
-        for p, dm in zip(self.plans, self.dms):
-            p.assemble(dm.coord, dm.coef, dm.values, dm.prec, precision=precision)
+        for p_, dm in zip(self.plans, self.dms):
+            p_.assemble(dm.coord, dm.coef, dm.values, dm.prec, precision=precision)
         gpu.synchronize()
 
+    def _register(self, mesh, dm, plan):
+        ncoef = mesh.nbElem * (2 if mesh.operatorID == OP_ELA else 1)
+        sm = ServerMesh(plan.ptr, dm.coord.ptr, dm.coef.ptr if dm.coef is not None else None,
+                        mesh.nnz * mesh.dim, mesh.nbNodes * mesh.dim, 3 * mesh.nbNodes, ncoef)
+        return check(self._lib.mf_server_add_mesh(self._srv, ctypes.byref(sm)),
+                     "mf_server_add_mesh")
+
+    def __del__(self):
+        if getattr(self, "_srv", None):
+            self._lib.mf_server_destroy(self._srv)
+            self._srv = None
+
     def add_mesh(self, mesh):
-        """Admits a new resident mesh while the server runs (its topology arrives with a
-        request stream of its own): device arrays, tiled plan and one warm launch on a slot
-        stream. The mesh's host setup (coloring, CSR, mask: minifem.Mesh) comes first.
-        Returns the mesh's index for requests."""
+        """Admits a new resident mesh while the server holds others (its topology arrives
+        with a request stream of its own): device arrays, tiled plan and one warm launch.
+        The mesh's setup (coloring, CSR, mask: minifem.Mesh, on the host or the GPU) comes
+        first. Returns the mesh's index for requests."""
         dm = gpu.DeviceMesh(mesh)
         plan = gpu.Plan(mesh, **self._plan_params)
-        s = self.slots[0]
+        s = gpu.Stream()
         plan.assemble(dm.coord, dm.coef, dm.values, dm.prec, precision=self.precision, stream=s)
         s.sync()
+        idx = self._register(mesh, dm, plan)
         self.meshes.append(mesh)
         self.dms.append(dm)
         self.plans.append(plan)
-        return len(self.meshes) - 1
+        assert idx == len(self.meshes) - 1
+        return idx
 
-    def _views(self, req, off):
-        m = self.meshes[req.mesh]
-        base = self.arena_mem.ptr + off
-        inp = gpu.DeviceView(base, req.host.size, req.host.dtype, self.arena_mem)
-        vo = base + (req.host.nbytes + _ALIGN - 1) // _ALIGN * _ALIGN
-        values = gpu.DeviceView(vo, m.nnz * m.dim, np.float64, self.arena_mem)
-        po = vo + (values.nbytes + _ALIGN - 1) // _ALIGN * _ALIGN
-        prec = gpu.DeviceView(po, m.nbNodes * m.dim, np.float64, self.arena_mem)
-        return inp, values, prec
-
-    def _footprint_aligned(self, req):
-        m = self.meshes[req.mesh]
-        a = lambda n: (n + _ALIGN - 1) // _ALIGN * _ALIGN
-        return a(req.host.nbytes) + a(8 * m.dim * m.nnz) + a(8 * m.dim * m.nbNodes)
+    def arena_used(self):
+        """Arena bytes held by requests (0 between runs)."""
+        return int(self._lib.mf_server_arena_used(self._srv))
 
     def _mode(self, req):
         """The precision mode of a request's launch: a float32 coefficient field is read as
@@ -206,66 +181,6 @@ class Server:
         if req.kind == "coef" and req.host.dtype == np.float32:
             return PREC_MIXED_F32COEF
         return self.precision
-
-    def _job(self, req, views):
-        inp, values, prec = views
-        dm = self.dms[req.mesh]
-        coord = inp if req.kind == "coord" else dm.coord
-        coef = inp if req.kind == "coef" else dm.coef
-        return (self.plans[req.mesh], coord, coef, values, prec)
-
-    def _group_fits(self, mode, mesh_ids):
-        """Whether the plans of these meshes run as one launch (their tiles fit one CU's LDS
-        together; the check reads only the plans)."""
-        key = (mode, tuple(sorted(set(mesh_ids))))
-        if key not in self._fits:
-            jobs = [(self.plans[i], self.dms[i].coord, None, self.dms[i].values, self.dms[i].prec)
-                    for i in key[1]]
-            self._fits[key] = gpu.plans_lds_bytes(jobs, mode) is not None
-        return self._fits[key]
-
-    def _launch(self, group, slot):
-        """One launch for a group of requests on one slot: their inputs uploaded on the slot's
-        stream, then one multi-plan launch over all of them."""
-        s = self.slots[slot]
-        jobs = []
-        for req, _, views in group:
-            gpu.copy_h2d_async(views[0], req.host, s)
-            jobs.append(self._job(req, views))
-        gpu.plans_assemble(jobs, self._mode(group[0][0]), stream=s)
-        self.events[slot].record(s)
-
-    def _take(self, pending, now, memory_waits):
-        """Admission for one free slot: requests that have arrived, in arrival order, that fit
-        the HBM budget, one precision mode per launch, at most max_group of them. A request
-        that does not fit waits; later ones may pass it (first fit) until it has been passed
-        max_bypass times, then admission holds until it fits."""
-        group, mode, i = [], None, 0
-        head = pending[0] if pending else None
-        while i < len(pending) and len(group) < self.max_group:
-            req = pending[i]
-            if req.t_arrival > now:
-                break
-            m = self._mode(req)
-            if mode is not None and (m != mode or not self._group_fits(
-                    m, [g[0].mesh for g in group] + [req.mesh])):
-                i += 1
-                continue
-            blk = self.arena.alloc(self._footprint_aligned(req))
-            if blk is None:
-                memory_waits.add(req.rid)
-                if req is head and self._bypassed >= self.max_bypass:
-                    break  # the head has waited long enough: nothing passes it
-                i += 1
-                continue
-            del pending[i]
-            group.append((req, blk, self._views(req, blk[0])))
-            mode = m
-        if head is not None and group and pending and pending[0] is head:
-            self._bypassed += 1  # the head stays behind the requests just admitted
-        elif not pending or pending[0] is not head:
-            self._bypassed = 0
-        return group
 
     def stage(self, requests):
         """Copies each request's payload into pinned host memory: the receive buffer a
@@ -279,66 +194,54 @@ class Server:
     def run(self, requests, keep_results=False):
         """Serves `requests` (arrival times relative to the start) and returns the stats;
         with keep_results, each request's (values, prec) copied back after it finishes."""
-        for r in requests:
+        n = len(requests)
+        arr = (RequestDesc * max(n, 1))()
+        results = {}
+        for i, r in enumerate(requests):
             if r.host is None:
                 raise ValueError("stage() the requests first")
-            if self._footprint_aligned(r) > self.arena.size:
-                raise ValueError(f"request {r.rid} needs more than the HBM budget")
-        pending = sorted(requests, key=lambda r: r.t_arrival)
-        busy = [None] * len(self.slots)  # per slot: [(request, arena block, views), ...]
-        results = {}
-        done = []
-        memory_waits = set()  # requests that found a slot free but not their HBM
-        launches = 0
-        self._bypassed = 0
-        t0 = time.perf_counter()
-        while pending or any(b is not None for b in busy):
-            for i, b in enumerate(busy):
-                if b is not None and self.events[i].query():
-                    t_done = time.perf_counter() - t0
-                    for req, blk, views in b:
-                        req.t_done = t_done
-                        if keep_results:
-                            results[req.rid] = (views[1].numpy(), views[2].numpy())
-                        self.arena.free(blk)
-                        done.append(req)
-                    busy[i] = None
-            now = time.perf_counter() - t0
-            for i in range(len(self.slots)):
-                if busy[i] is not None or not pending or pending[0].t_arrival > now:
-                    continue
-                group = self._take(pending, now, memory_waits)
-                if not group:
-                    break  # nothing admissible until memory frees up or more arrives
-                for req, _, _ in group:
-                    req.t_start = now
-                self._launch(group, i)
-                busy[i] = group
-                launches += 1
-            time.sleep(0)  # yield, then poll again
-        makespan = max(r.t_done for r in done) - min(r.t_arrival for r in done)
-        lat = np.array([r.t_done - r.t_arrival for r in done])
-        blocks = sum(self.meshes[r.mesh].nnz for r in done)
-        stats = {"requests": len(done), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            vo = po = None
+            if keep_results:
+                m = self.meshes[r.mesh]
+                results[r.rid] = (np.empty(m.nnz * m.dim), np.empty(m.nbNodes * m.dim))
+                vo, po = results[r.rid][0].ctypes.data, results[r.rid][1].ctypes.data
+            arr[i] = RequestDesc(r.mesh, REQ_COEF if r.kind == "coef" else REQ_COORD,
+                                 self._mode(r), 0, r.host.ptr, r.host.nbytes,
+                                 float(r.t_arrival), vo, po)
+        t_start, t_done = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        st = ServerStats()
+        rc = self._lib.mf_server_run(self._srv, ctypes.cast(arr, ctypes.c_void_p), n,
+                                     t_start.ctypes.data, t_done.ctypes.data, ctypes.byref(st))
+        if rc == MF_ERR_BUDGET:
+            raise ValueError("a request needs more than the HBM budget")
+        check(rc, "mf_server_run")
+        for i, r in enumerate(requests):
+            r.t_start, r.t_done = float(t_start[i]), float(t_done[i])
+        lat = np.array([r.t_done - r.t_arrival for r in requests])
+        blocks = sum(self.meshes[r.mesh].nnz for r in requests)
+        makespan = float(st.makespan_s)
+        stats = {"requests": n, "p50_ms": float(np.percentile(lat, 50) * 1e3),
                  "p99_ms": float(np.percentile(lat, 99) * 1e3),
-                 "mean_ms": float(lat.mean() * 1e3), "makespan_s": float(makespan),
+                 "mean_ms": float(lat.mean() * 1e3), "makespan_s": makespan,
                  "blocks_per_s": float(blocks / makespan),
-                 "requests_per_s": float(len(done) / makespan),
-                 "memory_waits": len(memory_waits), "launches": launches,
-                 "mean_requests_per_launch": len(done) / max(launches, 1)}
+                 "requests_per_s": float(n / makespan),
+                 "memory_waits": int(st.memory_waits), "launches": int(st.launches),
+                 "mean_requests_per_launch": n / max(int(st.launches), 1)}
         return (stats, results) if keep_results else stats
 
     def run_alone(self, req):
         """One request with the device to itself, through the single-plan launch
-        (mf_plan_assemble): (values, prec)."""
-        blk = self.arena.alloc(self._footprint_aligned(req))
-        try:
-            views = self._views(req, blk[0])
-            s = self.slots[0]
-            gpu.copy_h2d_async(views[0], req.host, s)
-            plan, coord, coef, values, prec = self._job(req, views)
-            plan.assemble(coord, coef, values, prec, precision=self._mode(req), stream=s)
-            s.sync()
-            return views[1].numpy(), views[2].numpy()
-        finally:
-            self.arena.free(blk)
+        (mf_plan_assemble), in buffers of its own: (values, prec)."""
+        m = self.meshes[req.mesh]
+        s = gpu.Stream()
+        inp = gpu.DeviceArray(req.host.size, req.host.dtype)
+        values = gpu.DeviceArray(m.nnz * m.dim, np.float64)
+        prec = gpu.DeviceArray(m.nbNodes * m.dim, np.float64)
+        gpu.copy_h2d_async(inp, req.host, s)
+        dm = self.dms[req.mesh]
+        coord = inp if req.kind == "coord" else dm.coord
+        coef = inp if req.kind == "coef" else dm.coef
+        self.plans[req.mesh].assemble(coord, coef, values, prec, precision=self._mode(req),
+                                      stream=s)
+        s.sync()
+        return values.numpy(), prec.numpy()

@@ -153,7 +153,7 @@ def test_serving_mix_equals_alone(mf, orc, gpu, max_group):
     stats, mixed = srv.run(reqs, keep_results=True)
     assert stats["requests"] == len(reqs)
     assert 0 < stats["p50_ms"] <= stats["p99_ms"]
-    assert srv.arena.used == 0
+    assert srv.arena_used() == 0
     for r in reqs:
         v, p = srv.run_alone(r)
         assert np.array_equal(v, mixed[r.rid][0]), r.rid
@@ -178,6 +178,35 @@ def test_serving_rejects_request_above_budget(mf, gpu):
     srv.stage(reqs)
     with pytest.raises(ValueError):
         srv.run(reqs)
+
+
+def test_serving_rejects_bad_requests(mf, gpu):
+    """The native loop checks every request before it serves any: a payload of the wrong
+    size (the kernel would read past it), an unknown mesh or kind, the float coefficient
+    mode on coordinates; nothing is admitted and the arena stays empty."""
+    import ctypes
+    from minifem._lib import MF_ERR_ARG, REQ_COEF, REQ_COORD, Request as RequestDesc
+    from minifem.serve import Server, make_requests, served_meshes
+    meshes = served_meshes(count=2, n_lo=2000, n_hi=3000, seed=1000)
+    srv = Server(meshes, hbm_budget=1 << 26, slots=2)
+    reqs = make_requests(meshes, count=2, rate_hz=1000.0)
+    srv.stage(reqs)
+    r = reqs[0]
+    lib = mf.lib()
+    good = (r.mesh, REQ_COEF if r.kind == "coef" else REQ_COORD, 0, 0, r.host.ptr,
+            r.host.nbytes, 0.0, None, None)
+    bad = [good[:5] + (r.host.nbytes - 8,) + good[6:],            # short payload
+           (5,) + good[1:],                                        # no such mesh
+           good[:1] + (7,) + good[2:],                             # no such kind
+           good[:1] + (REQ_COORD, 2) + good[3:]]                   # float coefficients on coords
+    for b in bad:
+        arr = (RequestDesc * 1)(RequestDesc(*b))
+        ts, td = np.zeros(1), np.zeros(1)
+        assert lib.mf_server_run(srv._srv, ctypes.cast(arr, ctypes.c_void_p), 1,
+                                 ts.ctypes.data, td.ctypes.data, None) == MF_ERR_ARG, b
+    assert srv.arena_used() == 0
+    stats = srv.run(reqs)  # and the server still serves
+    assert stats["requests"] == 2 and srv.arena_used() == 0
 
 
 def test_batched_full_config(mf, orc, gpu):
@@ -249,7 +278,7 @@ def test_served_full_config(mf, orc, gpu, precision):
     srv.stage(reqs)
     stats, mixed = srv.run(reqs, keep_results=True)
     print(f"[serve full] {stats}", flush=True)
-    assert stats["requests"] == len(reqs) and srv.arena.used == 0
+    assert stats["requests"] == len(reqs) and srv.arena_used() == 0
     assert stats["launches"] < len(reqs)  # requests that arrived together shared launches
     assert 0 < stats["p50_ms"] <= stats["p99_ms"]
     for r in reqs:

@@ -1,42 +1,9 @@
-"""Host logic of the served form that needs no GPU: the HBM arena (first fit, 256-byte
-blocks, merging of freed neighbours), the precision modes a Server accepts, and request
-re-timing."""
+"""Host logic of the served form that needs no GPU: the precision modes a Server accepts,
+request re-timing, and the request descriptors the native loop reads."""
 import numpy as np
 import pytest
 
-from minifem.serve import Arena, Request, Server, retime
-
-
-def test_arena_first_fit_and_merge():
-    a = Arena(4096)
-    b1 = a.alloc(100)   # rounded to 256
-    b2 = a.alloc(1000)  # 1024
-    b3 = a.alloc(256)
-    assert b1 == (0, 256) and b2 == (256, 1024) and b3 == (1280, 256)
-    assert a.used == 1536
-    assert a.alloc(4096) is None
-    a.free(b2)
-    assert a.alloc(2000) == (1536, 2048)  # the hole of 1024 is too small: first fit after it
-    a.free(b1)
-    a.free(b3)  # b1, b2 and b3 merge back into one block at 0
-    assert a._free[0] == (0, 1536)
-    assert a.alloc(1536) == (0, 1536)
-
-
-def test_arena_returns_to_empty():
-    a = Arena(1 << 20)
-    rng = np.random.default_rng(3)
-    blocks = []
-    for _ in range(200):
-        if blocks and rng.random() < 0.45:
-            a.free(blocks.pop(int(rng.integers(len(blocks)))))
-        else:
-            b = a.alloc(int(rng.integers(1, 20000)))
-            if b is not None:
-                blocks.append(b)
-    for b in blocks:
-        a.free(b)
-    assert a.used == 0 and a._free == [(0, 1 << 20)]
+from minifem.serve import Request, Server, retime
 
 
 def test_server_rejects_f32coef_precision():
@@ -55,3 +22,41 @@ def test_retime_keeps_payloads():
     assert all(q.host is r.host and q.mesh == r.mesh for q, r in zip(out, reqs))
     t = np.array([q.t_arrival for q in out])
     assert np.all(np.diff(t) > 0) and t[-1] < 0.01  # 50 arrivals at 100,000/s
+
+
+def test_request_descriptor_layout(tmp_path):
+    """mf_request, mf_server_mesh and mf_server_stats in ctypes match include/minifem.h:
+    sizes and field offsets printed by a C program built against the header."""
+    import ctypes
+    import os
+    import shutil
+    import subprocess
+    from minifem._lib import Request as RequestDesc, ServerMesh, ServerStats
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    inc = os.path.join(os.path.dirname(__file__), "..", "include")
+    src = tmp_path / "layout.c"
+    src.write_text("""#include <stddef.h>
+#include <stdio.h>
+#include "minifem.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(mf_request), offsetof(mf_request, payload),
+         offsetof(mf_request, payload_bytes), offsetof(mf_request, t_arrival),
+         offsetof(mf_request, values_out), offsetof(mf_request, prec_out));
+  printf("%zu %zu %zu %zu %zu\\n", sizeof(mf_server_mesh), offsetof(mf_server_mesh, nvalues),
+         offsetof(mf_server_mesh, nprec), offsetof(mf_server_mesh, ncoord),
+         offsetof(mf_server_mesh, ncoef));
+  printf("%zu %zu\\n", sizeof(mf_server_stats), offsetof(mf_server_stats, makespan_s));
+  return 0;
+}
+""")
+    exe = tmp_path / "layout"
+    subprocess.check_call([cc, "-I", inc, str(src), "-o", str(exe)])
+    lines = [list(map(int, ln.split())) for ln in subprocess.check_output([str(exe)]).decode().split("\n") if ln]
+    R = RequestDesc
+    assert lines[0] == [ctypes.sizeof(R), R.payload.offset, R.payload_bytes.offset,
+                        R.t_arrival.offset, R.values_out.offset, R.prec_out.offset]
+    assert lines[1] == [ctypes.sizeof(ServerMesh), ServerMesh.nvalues.offset,
+                        ServerMesh.nprec.offset, ServerMesh.ncoord.offset, ServerMesh.ncoef.offset]
+    assert lines[2] == [ctypes.sizeof(ServerStats), ServerStats.makespan_s.offset]
