@@ -209,6 +209,31 @@ def test_serving_rejects_bad_requests(mf, gpu):
     assert stats["requests"] == 2 and srv.arena_used() == 0
 
 
+@pytest.mark.parametrize("max_bypass", [0, 3])
+def test_serving_bypass_is_bounded(mf, gpu, max_bypass):
+    """Admission policy of the native loop. A large request that fits the arena only when
+    it is alone arrives second, with twenty small ones behind it, all at once. It waits
+    for memory; exactly max_bypass small requests are admitted past it (first fit), then
+    admission holds until it fits; every request is served and the arena ends empty."""
+    from minifem.serve import Request, Server
+    big = mf.Mesh.cube(31, 0.2, 1, mf.OP_ELA, coef_seed=5)
+    small = mf.Mesh.cube(8, 0.2, 2, mf.OP_ELA, coef_seed=6)
+    fp = lambda m: sum((n + 255) // 256 * 256 for n in (8 * m.coef.size, 72 * m.nnz, 72 * m.nbNodes))
+    srv = Server([big, small], hbm_budget=fp(big) + fp(small) // 2, slots=4, max_group=1,
+                 max_bypass=max_bypass)
+    reqs = [Request(0, 0.0, 1, "coef", small.coef), Request(1, 0.0, 0, "coef", big.coef)]
+    reqs += [Request(2 + i, 0.0, 1, "coef", small.coef) for i in range(20)]
+    srv.stage(reqs)
+    stats = srv.run(reqs)
+    assert stats["requests"] == len(reqs) and srv.arena_used() == 0
+    assert stats["memory_waits"] >= 1
+    t_big = reqs[1].t_start
+    passed = [r.rid for r in reqs[2:] if r.t_start < t_big]
+    print(f"[serve bypass] max_bypass {max_bypass}: passed {passed}", flush=True)
+    assert len(passed) == max_bypass
+    assert all(r.t_done >= r.t_start >= 0 for r in reqs)
+
+
 def test_batched_full_config(mf, orc, gpu):
     """The batched workload at its size: 64 jittered cubes of 5k-200k nodes, lap and ela
     alternating (batch_meshes() defaults). The per-mesh form (one launch per mesh) is the
