@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <thread>
 #include <vector>
@@ -283,8 +284,20 @@ struct Emitter {
     }
 
     // Tile ti appended to o.
+    // Per-row scratch of the task records: for each column of the row, the elements
+    // holding both nodes (the first kListLen as contributions), their incidence mask and
+    // count, filled in one pass over the row's incidence list.
+    static constexpr int kMaxRowLen = 256;  // the task record's 8-bit column offsets
+    struct ColScratch {
+        SmallMap local{10};  // column node -> its position in the row
+        uint16_t list[kMaxRowLen][kListLen];
+        uint64_t mask[kMaxRowLen];
+        int cnt[kMaxRowLen];
+    };
+
+    // Tile ti appended to o.
     int emit(int ti, TileChunk &o, SmallMap &nodeLocal, SmallMap &elemLocal,
-             std::vector<int> &rows, std::vector<int> &tel) const
+             std::vector<int> &rows, std::vector<int> &tel, ColScratch &cs) const
     {
         const Leaf &lf = leaves[ti];
         rows.assign(order.begin() + lf.first, order.begin() + lf.second);
@@ -358,6 +371,35 @@ struct Emitter {
                     if (e2n[(int64_t)e * 4 + k] - 1 == a) va = k;
                 o.inc.push_back((uint16_t)(elemLocal.get(e) << 2 | va));
             }
+            // the elements holding a and each column c, ascending: one pass over a's
+            // incidence list, each element filed under the columns of its other vertices
+            // (a repeated vertex counts once, with its last slot, for a and for c)
+            if (len > kMaxRowLen) return MF_ERR_ARG;
+            cs.local.clear();
+            for (int q = 0; q < len; q++) {
+                cs.local.insert(col[row[a] + q] - 1, q);
+                cs.cnt[q] = 0;
+                cs.mask[q] = 0;
+            }
+            for (int j = n2eI[a]; j < n2eI[a + 1]; j++) {
+                const int *v = e2n + (int64_t)n2eV[j] * 4;
+                int va = -1;
+                for (int k = 0; k < 4; k++)
+                    if (v[k] - 1 == a) va = k;
+                const int tl = elemLocal.get(n2eV[j]);
+                for (int k = 0; k < 4; k++) {
+                    const int c = v[k] - 1;
+                    if (c == a) continue;
+                    bool later = false;
+                    for (int k2 = k + 1; k2 < 4; k2++) later |= v[k2] == v[k];
+                    if (later) continue;
+                    const int q = cs.local.get(c);
+                    if (q < 0) return MF_ERR_ARG;  // a vertex outside the row's pattern
+                    if (j - n2eI[a] < kMaskValence) cs.mask[q] |= 1ull << (j - n2eI[a]);
+                    if (cs.cnt[q] < kListLen) cs.list[q][cs.cnt[q]] = (uint16_t)(tl << 4 | va << 2 | k);
+                    cs.cnt[q]++;
+                }
+            }
             for (int q = 0; q < len; q++) {
                 const int c = col[row[a] + q] - 1;
                 if (c < 0 || c >= N || nodeLocal.get(c) < 0 || q > 0xFF) return MF_ERR_ARG;
@@ -373,31 +415,17 @@ struct Emitter {
                     oc = (int)(pc - row[c]);
                     if (oc > 0xFF) return MF_ERR_ARG;
                 }
-                // the elements holding a and c, ascending, with a's and c's slots
-                uint64_t m = 0;
-                uint16_t list[kListLen];
-                int cnt = 0;
-                for (int j = n2eI[a]; j < n2eI[a + 1]; j++) {
-                    const int *v = e2n + (int64_t)n2eV[j] * 4;
-                    int va = -1, vc = -1;
-                    for (int k = 0; k < 4; k++) {
-                        if (v[k] - 1 == a) va = k;
-                        if (v[k] - 1 == c) vc = k;
-                    }
-                    if (vc < 0) continue;
-                    if (j - n2eI[a] < kMaskValence) m |= 1ull << (j - n2eI[a]);
-                    if (cnt < kListLen) list[cnt] = (uint16_t)(elemLocal.get(n2eV[j]) << 4 | va << 2 | vc);
-                    cnt++;
-                }
+                const int cnt = cs.cnt[q];
                 if (cnt < 1) return MF_ERR_ARG;
                 if (cnt <= kListLen) {
-                    for (int z = cnt; z < kListLen; z++) list[z] = (uint16_t)(TE << 4);
+                    uint16_t list[kListLen];
+                    for (int z = 0; z < kListLen; z++) list[z] = z < cnt ? cs.list[q][z] : (uint16_t)(TE << 4);
                     for (int w = 0; w < 3; w++)
                         o.task_rec.push_back((uint32_t)list[2 * w] | (uint32_t)list[2 * w + 1] << 16);
                 } else if (val <= kMaskValence) {
                     o.task_rec.push_back(kMaskRecord);
-                    o.task_rec.push_back((uint32_t)m);
-                    o.task_rec.push_back((uint32_t)(m >> 32));
+                    o.task_rec.push_back((uint32_t)cs.mask[q]);
+                    o.task_rec.push_back((uint32_t)(cs.mask[q] >> 32));
                 } else {
                     o.task_rec.push_back(kWalkRecord);
                     o.task_rec.push_back(0u);
@@ -425,8 +453,9 @@ struct Emitter {
         try {
             SmallMap nodeLocal(10), elemLocal(12);
             std::vector<int> rows, tel;
+            auto cs = std::make_unique<ColScratch>();
             for (int ti = t0; ti < t1 && o.status == MF_OK; ti++)
-                o.status = emit(ti, o, nodeLocal, elemLocal, rows, tel);
+                o.status = emit(ti, o, nodeLocal, elemLocal, rows, tel, *cs);
         } catch (...) {
             o.status = MF_ERR_NOMEM;
         }
