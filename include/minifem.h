@@ -421,7 +421,8 @@ int mf_server_add_mesh(mf_server *server, const mf_server_mesh *mesh);
  * more than the arena; MF_ERR_ARG for a bad mesh, kind, mode or payload size. */
 int mf_server_run(mf_server *server, const mf_request *reqs, int n, double *t_start,
                   double *t_done, mf_server_stats *stats);
-/* Arena bytes held by requests (0 between runs). */
+/* One run at a time per server (the loop is single-threaded; two threads running one
+ * server race). Arena bytes held by requests (0 between runs): */
 int64_t mf_server_arena_used(const mf_server *server);
 int mf_server_destroy(mf_server *server);
 

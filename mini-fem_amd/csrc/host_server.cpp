@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cstdint>
 #include <map>
+#include <new>
 #include <thread>
 #include <utility>
 #include <vector>
@@ -85,7 +86,10 @@ struct mf_server {
     std::vector<mf_server_mesh> meshes;
     std::map<std::pair<int, std::vector<int>>, bool> fits;  // (mode, mesh ids) -> one launch
 
-    mf_server(void *mem, int64_t bytes) : arena_mem(static_cast<unsigned char *>(mem)), arena(bytes) {}
+    mf_server(void *mem, int64_t bytes)
+        : arena_mem(static_cast<unsigned char *>(mem)), arena(bytes)
+    {
+    }
     ~mf_server()
     {
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
@@ -234,98 +238,108 @@ extern "C" int mf_server_run(mf_server *server, const mf_request *reqs, int n, d
         };
         double first_arrival = n > 0 ? reqs[pending[0]].t_arrival : 0.0, last_done = 0.0;
         std::vector<mf_plan_job> jobs;
-        while (done_count < n) {
-            bool progress = false;
-            // 1: completions free their slot and arena blocks
-            for (int k = 0; k < slots; k++) {
-                if (busy[(size_t)k].empty()) continue;
-                const hipError_t q = hipEventQuery(S.events[(size_t)k]);
-                if (q == hipErrorNotReady) continue;
-                if (q != hipSuccess) return fail(MF_ERR_HIP);
-                const double t = now();
-                for (const Admitted &a : busy[(size_t)k]) {
-                    t_done[a.req] = t;
-                    S.arena.release(a.off, a.n);
-                    done_count++;
-                }
-                last_done = t;
-                busy[(size_t)k].clear();
-                progress = true;
-            }
-            // 2: free slots take the requests that have arrived, in arrival order: one
-            // precision mode per launch, plans that fit one launch, memory first fit
-            const double tn = now();
-            while (head < pending.size() && taken[(size_t)pending[head]]) head++;
-            for (int k = 0; k < slots; k++) {
-                if (!busy[(size_t)k].empty()) continue;
-                if (head >= pending.size() || reqs[pending[head]].t_arrival > tn) break;
-                group.clear();
-                std::vector<int> ids;
-                int mode = -1;
-                const int hreq = pending[head];
-                bool head_taken = false;
-                for (size_t p = head; p < pending.size() && (int)group.size() < S.max_group; p++) {
-                    const int ri = pending[p];
-                    if (taken[(size_t)ri]) continue;
-                    const mf_request &r = reqs[ri];
-                    if (r.t_arrival > tn) break;
-                    if (mode >= 0) {
-                        if (r.mode != mode) continue;
-                        ids.push_back(r.mesh);
-                        const bool ok = S.group_fits(mode, ids);
-                        ids.pop_back();
-                        if (!ok) continue;
+        try {
+            while (done_count < n) {
+                bool progress = false;
+                // 1: completions free their slot and arena blocks
+                for (int k = 0; k < slots; k++) {
+                    if (busy[(size_t)k].empty()) continue;
+                    const hipError_t q = hipEventQuery(S.events[(size_t)k]);
+                    if (q == hipErrorNotReady) continue;
+                    if (q != hipSuccess) return fail(MF_ERR_HIP);
+                    const double t = now();
+                    for (const Admitted &a : busy[(size_t)k]) {
+                        t_done[a.req] = t;
+                        S.arena.release(a.off, a.n);
+                        done_count++;
                     }
-                    const int64_t need = S.footprint(r);
-                    const int64_t off = S.arena.alloc(need);
-                    if (off < 0) {
-                        if (!waited[(size_t)ri]) {
-                            waited[(size_t)ri] = 1;
-                            memory_waits++;
-                        }
-                        if (ri == hreq && bypassed >= S.max_bypass) break;  // the head waited long enough
-                        continue;
-                    }
-                    taken[(size_t)ri] = 1;
-                    if (ri == hreq) head_taken = true;
-                    group.push_back({ri, off, need});
-                    ids.push_back(r.mesh);
-                    mode = r.mode;
+                    last_done = t;
+                    busy[(size_t)k].clear();
+                    progress = true;
                 }
-                if (group.empty()) break;  // nothing admissible until memory frees up or more arrives
-                if (head_taken) bypassed = 0;
-                else bypassed++;  // the head stays behind the requests just admitted
-                hipStream_t st = S.streams[(size_t)k];
-                jobs.clear();
-                for (const Admitted &a : group) {
-                    const mf_request &r = reqs[a.req];
-                    t_start[a.req] = tn;
-                    if (hipMemcpyAsync(S.arena_mem + a.off, r.payload, (size_t)r.payload_bytes,
-                                       hipMemcpyHostToDevice, st) != hipSuccess)
-                        return fail(MF_ERR_HIP);
-                    jobs.push_back(S.job(r, a.off));
-                }
-                const int stl = mf_plans_assemble(jobs.data(), (int)jobs.size(), mode, st);
-                if (stl != MF_OK) return fail(stl);
-                for (size_t g = 0; g < group.size(); g++) {
-                    const mf_request &r = reqs[group[g].req];
-                    const mf_server_mesh &m = S.meshes[(size_t)r.mesh];
-                    if ((r.values_out &&
-                         hipMemcpyAsync(r.values_out, jobs[g].values, (size_t)(8 * m.nvalues),
-                                        hipMemcpyDeviceToHost, st) != hipSuccess) ||
-                        (r.prec_out &&
-                         hipMemcpyAsync(r.prec_out, jobs[g].prec, (size_t)(8 * m.nprec),
-                                        hipMemcpyDeviceToHost, st) != hipSuccess))
-                        return fail(MF_ERR_HIP);
-                }
-                if (hipEventRecord(S.events[(size_t)k], st) != hipSuccess) return fail(MF_ERR_HIP);
-                busy[(size_t)k].swap(group);
-                group.clear();
-                launches++;
-                progress = true;
+                // 2: free slots take the requests that have arrived, in arrival order: one
+                // precision mode per launch, plans that fit one launch, memory first fit
+                const double tn = now();
                 while (head < pending.size() && taken[(size_t)pending[head]]) head++;
+                for (int k = 0; k < slots; k++) {
+                    if (!busy[(size_t)k].empty()) continue;
+                    if (head >= pending.size() || reqs[pending[head]].t_arrival > tn) break;
+                    group.clear();
+                    std::vector<int> ids;
+                    int mode = -1;
+                    const int hreq = pending[head];
+                    bool head_taken = false;
+                    for (size_t p = head; p < pending.size() && (int)group.size() < S.max_group;
+                         p++) {
+                        const int ri = pending[p];
+                        if (taken[(size_t)ri]) continue;
+                        const mf_request &r = reqs[ri];
+                        if (r.t_arrival > tn) break;
+                        if (mode >= 0) {
+                            if (r.mode != mode) continue;
+                            ids.push_back(r.mesh);
+                            const bool ok = S.group_fits(mode, ids);
+                            ids.pop_back();
+                            if (!ok) continue;
+                        }
+                        const int64_t need = S.footprint(r);
+                        const int64_t off = S.arena.alloc(need);
+                        if (off < 0) {
+                            if (!waited[(size_t)ri]) {
+                                waited[(size_t)ri] = 1;
+                                memory_waits++;
+                            }
+                            // the head has waited long enough: nothing passes it
+                            if (ri == hreq && bypassed >= S.max_bypass) break;
+                            continue;
+                        }
+                        taken[(size_t)ri] = 1;
+                        if (ri == hreq) head_taken = true;
+                        group.push_back({ri, off, need});
+                        ids.push_back(r.mesh);
+                        mode = r.mode;
+                    }
+                    // nothing admissible until memory frees up or more requests arrive
+                    if (group.empty()) break;
+                    if (head_taken) bypassed = 0;
+                    else bypassed++;  // the head stays behind the requests just admitted
+                    hipStream_t st = S.streams[(size_t)k];
+                    jobs.clear();
+                    for (const Admitted &a : group) {
+                        const mf_request &r = reqs[a.req];
+                        t_start[a.req] = tn;
+                        if (hipMemcpyAsync(S.arena_mem + a.off, r.payload, (size_t)r.payload_bytes,
+                                           hipMemcpyHostToDevice, st) != hipSuccess)
+                            return fail(MF_ERR_HIP);
+                        jobs.push_back(S.job(r, a.off));
+                    }
+                    const int stl = mf_plans_assemble(jobs.data(), (int)jobs.size(), mode, st);
+                    if (stl != MF_OK) return fail(stl);
+                    for (size_t g = 0; g < group.size(); g++) {
+                        const mf_request &r = reqs[group[g].req];
+                        const mf_server_mesh &m = S.meshes[(size_t)r.mesh];
+                        if ((r.values_out &&
+                             hipMemcpyAsync(r.values_out, jobs[g].values, (size_t)(8 * m.nvalues),
+                                            hipMemcpyDeviceToHost, st) != hipSuccess) ||
+                            (r.prec_out &&
+                             hipMemcpyAsync(r.prec_out, jobs[g].prec, (size_t)(8 * m.nprec),
+                                            hipMemcpyDeviceToHost, st) != hipSuccess))
+                            return fail(MF_ERR_HIP);
+                    }
+                    if (hipEventRecord(S.events[(size_t)k], st) != hipSuccess)
+                        return fail(MF_ERR_HIP);
+                    busy[(size_t)k].swap(group);
+                    group.clear();
+                    launches++;
+                    progress = true;
+                    while (head < pending.size() && taken[(size_t)pending[head]]) head++;
+                }
+                if (!progress) std::this_thread::yield();
             }
-            if (!progress) std::this_thread::yield();
+        } catch (const std::bad_alloc &) {  // a host container: nothing may stay in flight
+            return fail(MF_ERR_NOMEM);
+        } catch (...) {
+            return fail(MF_ERR_STATE);
         }
         if (stats) {
             stats->launches = launches;
