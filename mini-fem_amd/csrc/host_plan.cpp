@@ -170,14 +170,43 @@ struct Partitioner {
             const double xa = c[(int64_t)a * 3 + ax], xb = c[(int64_t)b * 3 + ax];
             return xa < xb || (xa == xb && a < b);
         };
-        const int mid = lo + (hi - lo) / 2;
-        std::nth_element(order.begin() + lo, order.begin() + mid, order.begin() + hi, less);
+        // A cut near position p of order[a, b): the range is partitioned at p along the
+        // axis, then the cut moves to the widest coordinate gap within w of p. On a layered
+        // (e.g. jittered structured) mesh that gap lies between two layers of nodes, so a
+        // tile's faces are flat and its halo thin: at 100^3, 1.71 instead of 1.97 staged
+        // elements per element for lap tiles, which also grow larger under the same node cap.
+        auto cut = [&](int a, int b, int p, int w) {
+            std::nth_element(order.begin() + a, order.begin() + p, order.begin() + b, less);
+            const int l = std::max(a + 1, p - w), r = std::min(b - 1, p + w);
+            if (r - l < 2) return p;
+            std::nth_element(order.begin() + a, order.begin() + l, order.begin() + p, less);
+            std::nth_element(order.begin() + p, order.begin() + r, order.begin() + b, less);
+            std::sort(order.begin() + l, order.begin() + r, less);
+            double best = -1;
+            int bi = p;
+            for (int i = l + 1; i < r; i++) {
+                const double g = c[(int64_t)order[i] * 3 + ax] - c[(int64_t)order[i - 1] * 3 + ax];
+                if (g > best) {
+                    best = g;
+                    bi = i;
+                }
+            }
+            return bi;
+        };
+        const int half = lo + (hi - lo) / 2;
+        int mid = cut(lo, hi, half, (hi - lo) / 8);
+        if (!(fits(lo, mid, s) && fits(mid, hi, s))) {
+            // the flat cut leaves a piece too big: the median may still give two tiles
+            // where the flat one would give more
+            std::nth_element(order.begin() + lo, order.begin() + half, order.begin() + hi, less);
+            if (fits(lo, half, s) && fits(half, hi, s)) mid = half;
+            else mid = cut(lo, hi, half, (hi - lo) / 8);
+        }
         // thirds can fit only when the first one has at most max_rows rows
         if (hi - lo >= 6 && (hi - lo) / 3 <= max_rows && !(fits(lo, mid, s) && fits(mid, hi, s))) {
-            const int t1 = lo + (hi - lo) / 3, t2 = lo + 2 * (hi - lo) / 3;
             std::vector<int> saved(order.begin() + lo, order.begin() + hi);
-            std::nth_element(order.begin() + lo, order.begin() + t1, order.begin() + hi, less);
-            std::nth_element(order.begin() + t1, order.begin() + t2, order.begin() + hi, less);
+            const int t1 = cut(lo, hi, lo + (hi - lo) / 3, (hi - lo) / 12);
+            const int t2 = cut(t1, hi, lo + 2 * (hi - lo) / 3, (hi - lo) / 12);
             if (fits(lo, t1, s) && fits(t1, t2, s) && fits(t2, hi, s)) {
                 out.emplace_back(lo, t1);
                 out.emplace_back(t1, t2);
