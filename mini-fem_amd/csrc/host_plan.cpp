@@ -149,9 +149,10 @@ struct Partitioner {
     // Recursive coordinate bisection along the longest axis. A region whose halves do
     // not fit but whose thirds do is cut in three: tiles of a third rather than a quarter
     // of it, i.e. fewer and larger tiles. The top levels run their halves on two threads.
-    void rcb(int lo, int hi, FitScratch &s, std::vector<Leaf> &out, int depth)
+    // known: whether order[lo, hi) fits, when the caller has tested it (-1: not tested)
+    void rcb(int lo, int hi, FitScratch &s, std::vector<Leaf> &out, int depth, int known = -1)
     {
-        if (hi - lo <= 1 || fits(lo, hi, s)) {
+        if (hi - lo <= 1 || known == 1 || (known < 0 && fits(lo, hi, s))) {
             if (hi > lo) out.emplace_back(lo, hi);
             return;
         }
@@ -193,17 +194,31 @@ struct Partitioner {
             }
             return bi;
         };
+        // whether both halves of a cut at m fit; fl/fr: each half's result (-1: not tested)
+        auto halves_fit = [&](int m, int &fl, int &fr) {
+            fl = fits(lo, m, s) ? 1 : 0;
+            fr = fl ? (fits(m, hi, s) ? 1 : 0) : -1;
+            return fl == 1 && fr == 1;
+        };
         const int half = lo + (hi - lo) / 2;
-        int mid = cut(lo, hi, half, (hi - lo) / 8);
-        if (!(fits(lo, mid, s) && fits(mid, hi, s))) {
+        int mid = cut(lo, hi, half, (hi - lo) / 8), fl, fr;
+        bool both = halves_fit(mid, fl, fr);
+        if (!both) {
             // the flat cut leaves a piece too big: the median may still give two tiles
             // where the flat one would give more
             std::nth_element(order.begin() + lo, order.begin() + half, order.begin() + hi, less);
-            if (fits(lo, half, s) && fits(half, hi, s)) mid = half;
-            else mid = cut(lo, hi, half, (hi - lo) / 8);
+            int hl, hr;
+            if (halves_fit(half, hl, hr)) {
+                mid = half;
+                fl = hl;
+                fr = hr;
+                both = true;
+            } else {
+                mid = cut(lo, hi, half, (hi - lo) / 8);  // the same halves as the first cut
+            }
         }
         // thirds can fit only when the first one has at most max_rows rows
-        if (hi - lo >= 6 && (hi - lo) / 3 <= max_rows && !(fits(lo, mid, s) && fits(mid, hi, s))) {
+        if (hi - lo >= 6 && (hi - lo) / 3 <= max_rows && !both) {
             std::vector<int> saved(order.begin() + lo, order.begin() + hi);
             const int t1 = cut(lo, hi, lo + (hi - lo) / 3, (hi - lo) / 12);
             const int t2 = cut(t1, hi, lo + 2 * (hi - lo) / 3, (hi - lo) / 12);
@@ -215,6 +230,7 @@ struct Partitioner {
             }
             std::copy(saved.begin(), saved.end(), order.begin() + lo);  // back to the halves
         }
+        // fl/fr hold for the halves of this cut: the recursion need not test them again
         if (depth < spawn_depth && hi - lo > 4 * max_rows) {
             std::vector<Leaf> left;
             bool spawned = false;
@@ -223,7 +239,7 @@ struct Partitioner {
                 t = std::thread([&] {
                     try {
                         FitScratch s2;
-                        rcb(lo, mid, s2, left, depth + 1);
+                        rcb(lo, mid, s2, left, depth + 1, fl);
                     } catch (...) {
                         failed = 1;
                     }
@@ -233,8 +249,8 @@ struct Partitioner {
             }
             std::vector<Leaf> right;
             try {
-                if (!spawned) rcb(lo, mid, s, left, depth + 1);
-                rcb(mid, hi, s, right, depth + 1);
+                if (!spawned) rcb(lo, mid, s, left, depth + 1, fl);
+                rcb(mid, hi, s, right, depth + 1, fr);
             } catch (...) {  // join the other half before anything leaves this frame
                 failed = 1;
             }
@@ -244,8 +260,8 @@ struct Partitioner {
             out.insert(out.end(), right.begin(), right.end());
             return;
         }
-        rcb(lo, mid, s, out, depth + 1);
-        rcb(mid, hi, s, out, depth + 1);
+        rcb(lo, mid, s, out, depth + 1, fl);
+        rcb(mid, hi, s, out, depth + 1, fr);
     }
 
     // Breadth-first order of the node graph (through shared elements), then greedy
