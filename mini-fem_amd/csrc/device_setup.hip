@@ -287,6 +287,13 @@ struct DevBuf {
     T *get() const { return static_cast<T *>(p); }
 };
 
+// Waits for the stream when it goes out of scope: declared after the DevBufs of a call, it
+// runs before they free their memory, on every return path.
+struct StreamDrain {
+    hipStream_t s;
+    ~StreamDrain() { (void)hipStreamSynchronize(s); }
+};
+
 inline int dev_alloc(DevBuf &b, size_t bytes)
 {
     return hipMalloc(&b.p, std::max<size_t>(bytes, 16)) == hipSuccess ? MF_OK : MF_ERR_NOMEM;
@@ -312,6 +319,7 @@ extern "C" int mf_dev_coloring_creation(const int *elemToNode, int *colorPart, i
     }
     const size_t nE = (size_t)nbElem, N = (size_t)nbNodes;
     DevBuf index, value, next, npre, cnt, fa, fb, masks, part, misc, keys, order, tmp;
+    const StreamDrain drain{s};  // nothing in flight when the buffers are freed
     int st = MF_OK;
     if (st == MF_OK) st = dev_alloc(index, (N + 1) * sizeof(int));
     if (st == MF_OK) st = dev_alloc(value, 4 * nE * sizeof(int));
@@ -485,6 +493,7 @@ int dev_permute_rows(T *tab, const int *perm, int n, int dim, hipStream_t s)
     if (n < 0 || dim <= 0 || (n > 0 && (!tab || !perm))) return MF_ERR_ARG;
     if (n == 0) return MF_OK;
     DevBuf tmp, bad;
+    const StreamDrain drain{s};
     if (dev_alloc(tmp, (size_t)n * dim * sizeof(T)) != MF_OK || dev_alloc(bad, sizeof(int)) != MF_OK)
         return MF_ERR_NOMEM;
     if (hipMemsetAsync(bad.p, 0, sizeof(int), s) != hipSuccess) return MF_ERR_HIP;
