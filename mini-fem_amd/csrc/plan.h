@@ -71,9 +71,11 @@ MF_HOST_DEVICE constexpr int max_tile_elems(int block)
 }
 // Work blocks of 64 lanes per tile. A diagonal block takes a compute wave; the other
 // compute waves (at least one, which also copies the output out while the diagonal waves
-// invert) take up to task_blocks_per_wave task blocks each: one for ela (a 3x3 block
-// per lane held in registers), two for lap (one double per lane).
-MF_HOST_DEVICE constexpr int task_blocks_per_wave(bool ela) { return ela ? 1 : 2; }
+// invert) take up to task_blocks_per_wave task blocks each, their results held in
+// registers until the output image is written: two, for lap (one double per lane) and for
+// ela (a 3x3 block per lane; two of them fit the 128 registers of a 1024-thread
+// workgroup's lane, and larger ela tiles then stage fewer halo elements).
+MF_HOST_DEVICE constexpr int task_blocks_per_wave(bool) { return 2; }
 MF_HOST_DEVICE constexpr bool tile_blocks_fit(bool ela, int block, int n_diag, int n_task_blocks)
 {
     return n_diag < compute_waves(block) &&
