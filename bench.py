@@ -190,8 +190,10 @@ def main():
     plan = None
     tuning = None
     if args.schedule == "tiled":
+        # a mixed step's tiles are sized by its float stage
         params = dict(tile_lds_bytes=args.tile_lds, block_threads=args.block,
-                      max_tile_rows=args.max_rows, id_chunks=args.id_chunks)
+                      max_tile_rows=args.max_rows, id_chunks=args.id_chunks,
+                      stage_precision=prec_mode)
         if args.tune:
             from minifem.tune import tune
             params, res = tune(mesh, dm, precision=prec_mode, save=False, log=log)
@@ -301,8 +303,15 @@ def main():
         del ov, opv
     elif prec_mode != minifem.PREC_FP64:
         # against one FP64 step of the same kernel (bit-identical to the oracle's COLORING
-        # order, a property the GPU tests hold at this size)
-        step(minifem.PREC_FP64)
+        # order, a property the GPU tests hold at this size); a plan sized for the float
+        # stage runs the mixed modes only, so the FP64 step gets a plan of its own
+        if plan is not None and plan.stage_precision != minifem.PREC_FP64:
+            p64 = gpu.Plan(mesh, **dict(params, stage_precision=minifem.PREC_FP64))
+            p64.assemble(dm.coord, dm.coef, dm.values, dm.prec, precision=minifem.PREC_FP64,
+                         stream=stream)
+            del p64
+        else:
+            step(minifem.PREC_FP64)
         stream.sync()
         v64, p64 = dm.values.numpy(), dm.prec.numpy()
         accuracy = {"vs": "FP64 step of the same schedule (bit-identical to the oracle)",
@@ -406,15 +415,15 @@ def batched_leg(args):
     t0 = time.time()
     meshes = batch_meshes()
     B = sum(algorithmic_bytes(m.nbElem, m.nbNodes, m.nnz, m.operatorID) for m in meshes)
-    u = Batch(meshes, union=True)
     out = {"workload": f"{len(meshes)} jittered cubes of 5k-200k nodes, lap/ela alternating: "
-                       f"{sum(m.nbNodes for m in meshes)} nodes, {u.blocks()} CSR blocks",
-           "form": "union (one plan per operator), captured in one HIP graph",
-           "launches_per_batch": u.launches()}
+                       f"{sum(m.nbNodes for m in meshes)} nodes, "
+                       f"{sum(m.nnz for m in meshes)} CSR blocks",
+           "form": "union (one plan per operator), captured in one HIP graph; the mixed "
+                   "batch's tiles sized by its float stage"}
     ref = None
     for name, prec in (("fp64", minifem.PREC_FP64), ("mixed", minifem.PREC_MIXED)):
-        u.set_precision(prec)
-        u.capture()
+        u = Batch(meshes, precision=prec, union=True).capture()
+        out["launches_per_batch"] = u.launches()
         ms = u.time_ms(iters=max(5, min(args.steps, 20)), warmup=args.warmup)
         gbs = B / (ms * 1e-3) / 1e9
         out[name] = {"ms_per_batch": round(ms, 4), "blocks_per_s": round(u.blocks() / (ms * 1e-3), 1),
@@ -426,10 +435,10 @@ def batched_leg(args):
             out[name]["max_rel_err_vs_fp64"] = {
                 "values": max(rel_fro(v, r[0]) for (v, _), r in zip(got, ref)),
                 "prec": max(rel_fro(p, r[1]) for (_, p), r in zip(got, ref))}
-        del got
+        del got, u
     out["mixed_vs_fp64_throughput"] = round(out["fp64"]["ms_per_batch"] / out["mixed"]["ms_per_batch"], 3)
     log(f"batched: {out} ({time.time() - t0:.1f}s)")
-    del u, ref
+    del ref
     return out
 
 

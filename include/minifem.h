@@ -288,6 +288,12 @@ typedef struct mf_plan_params {
     int id_chunks;        /* > 1: bisect each of that many contiguous node-id ranges on its
                              own, so the tiles that run together write nearby CSR rows, at
                              the price of more halo elements per tile (default 1) */
+    int stage_precision;  /* the precision mode the tiles are sized for. MF_PREC_FP64 (0,
+                             default): the plan runs every mode. MF_PREC_MIXED or
+                             MF_PREC_MIXED_F32COEF: tiles sized by the float stage, which
+                             holds twice the values per byte (ela 100^3: 18 % fewer tiles);
+                             the plan then runs the two mixed modes only, and FP64 launches
+                             of it (mf_plan_assemble, mf_plans_assemble) return MF_ERR_ARG */
 } mf_plan_params;
 
 /* Host arrays: elemToNode (color-permuted, 1-based, 4nE), row[N+1], col[nnz],
@@ -313,8 +319,8 @@ int mf_plan_host_stats(int operatorID, int nbElem, int nbNodes, int nnz,
 
 /* Plan statistics: out[0] tiles, [1] tile-element slots, [2] off-diagonal tasks
  * (a pair computed once counts once), [3] incidence entries, [4] plan bytes on the
- * device, [5] LDS bytes per workgroup, [6] threads per workgroup, [7] rows per tile
- * (max). */
+ * device, [5] LDS bytes per workgroup (in the stage precision), [6] threads per
+ * workgroup, [7] rows per tile (max). */
 int mf_plan_stats(const mf_plan *plan, int64_t out[8]);
 
 /* One iteration: values (assembly, src/FEM.cc:183) and the inverted block-Jacobi prec

@@ -25,6 +25,7 @@ struct mf_plan {
     int lds_fp64 = 0, lds_mixed = 0;
     int back_fp64 = 0, back_mixed = 0;  // the largest tile's LDS past the fixed front part
     int n_isolated = 0;
+    bool float_stage = false;  // tiles sized for the float stage: the mixed modes only
     int grid[3] = {0, 0, 0};  // persistent grid per precision mode, set at creation
     int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     mf::TileHdr *tiles = nullptr;
@@ -1075,7 +1076,7 @@ int prepare_variant(mf_plan *p)
 template <int OP>
 int prepare_all(mf_plan *p)
 {
-    int st = prepare_variant<OP, MF_PREC_FP64>(p);
+    int st = p->float_stage ? MF_OK : prepare_variant<OP, MF_PREC_FP64>(p);
     if (st == MF_OK) st = prepare_variant<OP, MF_PREC_MIXED>(p);
     if (st == MF_OK) st = prepare_variant<OP, MF_PREC_MIXED_F32COEF>(p);
     return st;
@@ -1094,6 +1095,7 @@ int launch_variant(mf_plan *p, const double *coord, const double *coef, double *
 
 struct Resolved {
     int budget, block, max_rows, slot64, slot32, id_chunks;
+    bool float_stage;  // size the tiles by the float stage slot (mixed modes only)
 };
 
 int resolve_params(int op, const mf_plan_params *params, Resolved *r)
@@ -1104,11 +1106,16 @@ int resolve_params(int op, const mf_plan_params *params, Resolved *r)
     r->block = 1024;
     r->max_rows = 254;
     r->id_chunks = 1;
+    r->float_stage = false;
     if (params) {
         if (params->tile_lds_bytes > 0) r->budget = params->tile_lds_bytes;
         if (params->block_threads > 0) r->block = params->block_threads;
         if (params->max_tile_rows > 0) r->max_rows = params->max_tile_rows;
         if (params->id_chunks > 0) r->id_chunks = params->id_chunks;
+        const int sp = params->stage_precision;
+        if (sp != MF_PREC_FP64 && sp != MF_PREC_MIXED && sp != MF_PREC_MIXED_F32COEF)
+            return MF_ERR_ARG;
+        r->float_stage = sp != MF_PREC_FP64;
     }
     r->slot64 = mf::stage_len(ela, true) * 8;
     r->slot32 = mf::stage_len(ela, false) * 4;
@@ -1123,10 +1130,13 @@ int build_checked(mf::PlanHost &h, int op, int nE, int N, int nnz, const int *e2
                   const Resolved &r, int *lds64, int *lds32, int *back64 = nullptr,
                   int *back32 = nullptr)
 {
+    // the stage slot the tiles are sized by: a float-staged plan's FP64 layout may exceed
+    // one CU's LDS, and that plan never launches in FP64
+    const int slot = r.float_stage ? r.slot32 : r.slot64;
     int st = mf::build_plan(h, op, nE, N, nnz, e2n, row, col, cb, coord, r.budget, r.max_rows,
-                            r.slot64, r.block, r.id_chunks);
+                            slot, r.block, r.id_chunks);
     if (st != MF_OK) return st;
-    st = mf::validate_plan(h, row, col, e2n, r.slot64);
+    st = mf::validate_plan(h, row, col, e2n, slot);
     if (st != MF_OK) return st;
     int m64 = 0, m32 = 0, b64 = 0, b32 = 0;
     for (const auto &t : h.tiles) {
@@ -1141,7 +1151,8 @@ int build_checked(mf::PlanHost &h, int op, int nE, int N, int nnz, const int *e2
         b64 = std::max(b64, L64.total - L64.stage);
         b32 = std::max(b32, L32.total - L32.stage);
     }
-    if (m64 + 16 > mf::kMaxLdsBytes) return MF_ERR_ARG;  // one row needs more than a CU has
+    if ((r.float_stage ? m32 : m64) + 16 > mf::kMaxLdsBytes)
+        return MF_ERR_ARG;  // one row needs more than a CU has
     *lds64 = m64 + 16;
     *lds32 = m32 + 16;
     if (back64) *back64 = b64;
@@ -1150,7 +1161,8 @@ int build_checked(mf::PlanHost &h, int op, int nE, int N, int nnz, const int *e2
 }
 
 // stats: tiles, staged tile elements, off-diagonal tasks, incidence entries, plan
-// bytes, LDS bytes per workgroup (FP64), block threads, largest tile (rows)
+// bytes, LDS bytes per workgroup (in the precision the tiles are sized for), block
+// threads, largest tile (rows)
 void fill_stats(const mf::PlanHost &h, int lds64, int block, int64_t out[8])
 {
     out[0] = (int64_t)h.tiles.size();
@@ -1212,6 +1224,7 @@ extern "C" int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nb
         p->back_fp64 = back64;
         p->back_mixed = back32;
         p->n_isolated = (int)h.isolated.size();
+        p->float_stage = r.float_stage;
         p->max_node = h.max_node;
         p->max_elem = h.max_elem;
         int rc = upload(&p->tiles, h.tiles);
@@ -1226,7 +1239,7 @@ extern "C" int mf_plan_create(mf_plan **plan, int operatorID, int nbElem, int nb
         if (rc == MF_OK)
             rc = operatorID == MF_OP_ELA ? prepare_all<MF_OP_ELA>(p) : prepare_all<MF_OP_LAP>(p);
         if (rc != MF_OK) return rc;  // the guard frees the plan
-        fill_stats(h, lds64, r.block, p->stats);
+        fill_stats(h, r.float_stage ? lds32 : lds64, r.block, p->stats);
         *plan = guard.release();
         return MF_OK;
     });
@@ -1248,7 +1261,7 @@ extern "C" int mf_plan_host_stats(int operatorID, int nbElem, int nbNodes, int n
         st = build_checked(h, operatorID, nbElem, nbNodes, nnz, elemToNode, row, col, nullptr,
                            coord, r, &lds64, &lds32);
         if (st != MF_OK) return st;
-        fill_stats(h, lds64, r.block, out);
+        fill_stats(h, r.float_stage ? lds32 : lds64, r.block, out);
         return MF_OK;
     });
 }
@@ -1293,6 +1306,7 @@ extern "C" int mf_plan_assemble(mf_plan *plan, const double *coord, const double
                                 void *stream)
 {
     if (!plan || !values || !prec || (plan->N > 0 && !coord)) return MF_ERR_ARG;
+    if (precision == MF_PREC_FP64 && plan->float_stage) return MF_ERR_ARG;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const bool lap = plan->op == MF_OP_LAP;
     switch (precision) {
@@ -1365,6 +1379,7 @@ int resolve_jobs(const mf_plan_job *jobs, int njobs, int precision, JobsLaunch &
         const mf_plan_job &jb = jobs[i];
         const mf_plan *p = jb.plan;
         if (!p || !jb.values || !jb.prec || (p->N > 0 && !jb.coord)) return MF_ERR_ARG;
+        if (precision == MF_PREC_FP64 && p->float_stage) return MF_ERR_ARG;
         if (L.block == 0) L.block = p->block;
         if (p->block != L.block) return MF_ERR_ARG;  // one workgroup shape per launch
         L.table.job[i] = make_job(p, jb.coord, jb.coef, jb.values, jb.prec, jb.singular);

@@ -137,7 +137,8 @@ class ServerStats(ctypes.Structure):
 
 class PlanParams(ctypes.Structure):
     _fields_ = [("tile_lds_bytes", ctypes.c_int), ("block_threads", ctypes.c_int),
-                ("max_tile_rows", ctypes.c_int), ("id_chunks", ctypes.c_int)]
+                ("max_tile_rows", ctypes.c_int), ("id_chunks", ctypes.c_int),
+                ("stage_precision", ctypes.c_int)]
 
 
 class MiniFemError(RuntimeError):

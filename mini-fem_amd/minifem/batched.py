@@ -97,11 +97,18 @@ class UnionMesh:
 
 
 class Batch:
-    """Device-resident meshes with their plans, and the captured batch graph."""
+    """Device-resident meshes with their plans, and the captured batch graph.
+
+    A batch built for a mixed precision mode sizes its tiles by the float stage (larger
+    tiles, mf_plan_params.stage_precision) unless plan_params names a stage_precision;
+    such a batch runs the mixed modes only."""
 
     def __init__(self, meshes, precision=PREC_FP64, branches=8, union=False, **plan_params):
         self.meshes = list(meshes)
         self.precision = precision
+        plan_params = dict(plan_params)
+        plan_params.setdefault("stage_precision", precision)
+        self.stage_precision = plan_params["stage_precision"]
         self.union = bool(union)
         self._where = None
         self._jobs = None  # union form: the job table of its one launch
@@ -137,7 +144,8 @@ class Batch:
             plans = [gpu.Plan(u.mesh, partition_coord=u.partition_coord, **params)
                      for u in self.unions]
             jobs = [(p, dm.coord, dm.coef, dm.values, dm.prec) for p, dm in zip(plans, self.dms)]
-            fits = [gpu.plans_lds_bytes(jobs, prec) for prec in (PREC_FP64, PREC_MIXED)]
+            modes = (PREC_MIXED,) if params["stage_precision"] != PREC_FP64 else (PREC_FP64, PREC_MIXED)
+            fits = [gpu.plans_lds_bytes(jobs, prec) for prec in modes]
             if all(f is not None for f in fits):
                 self._jobs = jobs
                 return plans
@@ -146,8 +154,12 @@ class Batch:
         raise RuntimeError("union plans do not fit one launch")
 
     def set_precision(self, precision):
-        """Switches the precision mode of every launch (the plans serve every mode); the
-        captured graph, if any, is dropped and captured again on the next launch()."""
+        """Switches the precision mode of every launch; the captured graph, if any, is
+        dropped and captured again on the next launch(). Plans sized for the FP64 stage
+        serve every mode, plans sized for the float stage the mixed modes only."""
+        if precision == PREC_FP64 and self.stage_precision != PREC_FP64:
+            raise ValueError("this batch's tiles are sized for the float stage (mixed modes "
+                             "only); build it with stage_precision=0 to run FP64")
         self.precision = precision
         self.graph = None
 

@@ -385,15 +385,17 @@ class Plan:
     (mf_plan_create). assemble() runs one fused iteration: values + inverted prec."""
 
     def __init__(self, mesh, tile_lds_bytes=0, block_threads=0, max_tile_rows=0,
-                 use_coord=True, partition_coord=None, id_chunks=0):
+                 use_coord=True, partition_coord=None, id_chunks=0, stage_precision=PREC_FP64):
         """partition_coord: coordinates used only to group nodes into tiles (default: the
-        mesh's own)."""
+        mesh's own). stage_precision: the precision mode the tiles are sized for; a plan
+        sized for a mixed mode (float stage, larger tiles) runs the mixed modes only."""
         from ._lib import PlanParams
         _ensure()
         self._lib = lib()
         self.mesh = mesh
+        self.stage_precision = int(stage_precision)
         params = PlanParams(int(tile_lds_bytes), int(block_threads), int(max_tile_rows),
-                            int(id_chunks))
+                            int(id_chunks), self.stage_precision)
         pc = None
         if use_coord:
             pc = mesh.coord if partition_coord is None else np.ascontiguousarray(

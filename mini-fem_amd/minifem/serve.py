@@ -112,7 +112,8 @@ def retime(requests, rate_hz, seed=5000):
 class Server:
     """Resident meshes + an HBM budget + `slots` streams, served by the native request loop
     (mf_server_run); each launch serves up to `max_group` requests at once (slots=1,
-    max_group=1: the plain one-at-a-time server)."""
+    max_group=1: the plain one-at-a-time server). A PREC_MIXED server sizes its plans'
+    tiles by the float stage unless plan_params names a stage_precision."""
 
     def __init__(self, meshes, hbm_budget, slots=4, precision=PREC_FP64, plan_params=None,
                  max_group=2, max_bypass=8):
@@ -127,6 +128,7 @@ class Server:
         self._srv = None
         self.dms = [gpu.DeviceMesh(m) for m in self.meshes]
         self._plan_params = dict(plan_params or {})
+        self._plan_params.setdefault("stage_precision", precision)
         self.plans = [gpu.Plan(m, **self._plan_params) for m in self.meshes]
         self.budget = int(hbm_budget)
         self.arena_mem = gpu.DeviceArray(self.budget, np.uint8)

@@ -246,13 +246,14 @@ class Mesh:
 
 
 def plan_host_stats(mesh, tile_lds_bytes=0, block_threads=0, max_tile_rows=0, use_coord=True,
-                    partition_coord=None, id_chunks=0):
+                    partition_coord=None, id_chunks=0, stage_precision=0):
     """Builds and validates the tiled gather schedule on the host (mf_plan_host_stats).
-    partition_coord: coordinates used only to group nodes into tiles (default: the mesh's)."""
+    partition_coord: coordinates used only to group nodes into tiles (default: the mesh's).
+    stage_precision: the precision mode the tiles are sized for (0: FP64)."""
     import ctypes
     from ._lib import PlanParams
     params = PlanParams(int(tile_lds_bytes), int(block_threads), int(max_tile_rows),
-                        int(id_chunks))
+                        int(id_chunks), int(stage_precision))
     out = np.zeros(8, np.int64)
     pc = mesh.coord if partition_coord is None else _c(partition_coord, np.float64).ravel()
     if pc.size != 3 * mesh.nbNodes:

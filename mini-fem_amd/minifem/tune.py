@@ -47,11 +47,15 @@ def time_plan(plan, dm, precision, iters, stream):
 
 
 def tune(mesh, dm, precision=0, candidates=None, iters=10, save=True, log=None):
-    """Times every candidate; returns (best_params, [(params, us, stats), ...])."""
+    """Times every candidate; returns (best_params, [(params, us, stats), ...]). For a
+    mixed precision mode the candidates' tiles are sized by the float stage unless a
+    candidate names its stage_precision."""
     cands = candidates or DEFAULT_CANDIDATES
     stream = gpu.Stream()
     results = []
     for c in cands:
+        c = dict(c)
+        c.setdefault("stage_precision", precision)
         t0 = time.time()
         try:
             plan = gpu.Plan(mesh, **c)

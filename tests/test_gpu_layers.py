@@ -259,10 +259,10 @@ def test_batched_full_config(mf, orc, gpu):
         assert np.array_equal(want[i][0], ov)
         assert np.array_equal(want[i][1], om.prec(ov)[0])
         del om, ov
-    u = Batch(meshes, union=True)
     for prec in (mf.PREC_FP64, mf.PREC_MIXED):
-        u.set_precision(prec)
-        u.capture()
+        # the mixed batch's tiles are sized by its float stage
+        u = Batch(meshes, precision=prec, union=True).capture()
+        assert u.launches() == 1  # both operators' unions in one persistent launch
         for rep in range(2):
             for dm in u.dms:
                 dm.values.fill(np.nan)
@@ -280,7 +280,7 @@ def test_batched_full_config(mf, orc, gpu):
             del got
             u.run_eager()
             gpu.synchronize()
-    assert u.launches() == 1  # both operators' unions in one persistent launch
+        del u
 
 
 @pytest.mark.parametrize("precision", [0, 1])

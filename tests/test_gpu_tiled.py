@@ -95,10 +95,13 @@ def test_tiled_high_valence_masks(mf, orc, gpu, op, n_fan):
 
 
 @pytest.mark.parametrize("op", [0, 1])
-def test_tiled_mixed(mf, orc, gpu, op):
-    print(f"[tiled mixed] op={op}", flush=True)
+@pytest.mark.parametrize("stage", [0, 1])
+def test_tiled_mixed(mf, orc, gpu, op, stage):
+    """Mixed precision on a plan sized for the FP64 stage and on one sized for the float
+    stage (larger tiles): both bit-exact against the oracle's mixed restatement."""
+    print(f"[tiled mixed] op={op} stage={stage}", flush=True)
     mesh, om = build(mf, orc, 20, op, coef_seed=55)
-    plan, dm = run_plan(gpu, mesh, precision=mf.PREC_MIXED)
+    plan, dm = run_plan(gpu, mesh, precision=mf.PREC_MIXED, stage_precision=stage)
     vals = dm.values.numpy()
     assert np.array_equal(vals, om.assemble(mixed=True))
     ov = om.assemble()
@@ -109,15 +112,16 @@ def test_tiled_mixed(mf, orc, gpu, op):
 
 
 @pytest.mark.parametrize("op", [0, 1])
-def test_tiled_mixed_f32_coefficients(mf, orc, gpu, op):
+@pytest.mark.parametrize("stage", [0, 2])
+def test_tiled_mixed_f32_coefficients(mf, orc, gpu, op, stage):
     """MF_PREC_MIXED_F32COEF takes the coefficient field as float: bit-exact against the
     oracle's mixed restatement on the coefficients rounded to float, and within 1e-6 of
-    FP64 on the original ones."""
-    print(f"[tiled mixed f32 coef] op={op}", flush=True)
+    FP64 on the original ones; on plans sized for the FP64 and for the float stage."""
+    print(f"[tiled mixed f32 coef] op={op} stage={stage}", flush=True)
     coord, e2n, code, nnz = mf.generate_cube(20, 0.2, 1, 0)
     coef = mf.generate_coef(op, e2n.size // 4, 55)
     mesh = mf.Mesh(coord, e2n, code, nnz, op, coef=coef)
-    plan = gpu.Plan(mesh)
+    plan = gpu.Plan(mesh, stage_precision=stage)
     dm = gpu.DeviceMesh(mesh)
     c32 = gpu.DeviceArray.from_numpy(mesh.coef.astype(np.float32))
     for _ in range(2):  # the second run must repeat the first bit for bit
@@ -280,7 +284,67 @@ def test_tiled_bit_exact_bench_workload(mf, orc, gpu):
     del ov, prec
     dm.values.fill(np.nan)
     plan.assemble(dm.coord, dm.coef, dm.values, dm.prec, precision=mf.PREC_MIXED)
-    assert np.array_equal(dm.values.numpy(), om.assemble(mixed=True))
+    mixed = om.assemble(mixed=True)
+    assert np.array_equal(dm.values.numpy(), mixed)
+    # the mixed benchmark's plan: tiles sized by the float stage, fewer and larger
+    p32 = gpu.Plan(mesh, tile_lds_bytes=155000, block_threads=1024, max_tile_rows=254,
+                   stage_precision=mf.PREC_MIXED)
+    assert p32.stats()["tiles"] < plan.stats()["tiles"]
+    dm.values.fill(np.nan)
+    dm.prec.fill(np.nan)
+    p32.assemble(dm.coord, dm.coef, dm.values, dm.prec, precision=mf.PREC_MIXED)
+    vals = dm.values.numpy()
+    assert np.array_equal(vals, mixed)
+    assert np.array_equal(dm.prec.numpy(), om.prec(vals)[0])
+
+
+@pytest.mark.parametrize("op", [0, 1])
+def test_float_stage_plan_runs_mixed_modes_only(mf, orc, gpu, op):
+    """A plan sized for the float stage: no more tiles than the FP64-sized plan of the
+    same mesh (fewer for ela; lap tiles at the default budget are bound by the row cap),
+    both mixed modes bit-identical to that plan's, and FP64 launches of it rejected with
+    MF_ERR_ARG (alone and in a multi-plan launch), leaving the outputs untouched."""
+    from minifem._lib import MiniFemError
+    print(f"[float stage] op={op}", flush=True)
+    mesh, _ = build(mf, orc, 40, op, jitter=0.25, seed=4, shuffle=11, coef_seed=8, oracle=False)
+    p64 = gpu.Plan(mesh)
+    p32 = gpu.Plan(mesh, stage_precision=mf.PREC_MIXED)
+    assert p32.stage_precision == mf.PREC_MIXED
+    t32, t64 = p32.stats()["tiles"], p64.stats()["tiles"]
+    assert t32 < t64 if op == 1 else t32 <= t64
+    dm = gpu.DeviceMesh(mesh)
+    c32 = gpu.DeviceArray.from_numpy(mesh.coef.astype(np.float32))
+    want = {}
+    for prec, coef in ((mf.PREC_MIXED, dm.coef), (mf.PREC_MIXED_F32COEF, c32)):
+        got = []
+        for plan in (p64, p32):
+            dm.values.fill(np.nan)
+            dm.prec.fill(np.nan)
+            plan.assemble(dm.coord, coef, dm.values, dm.prec, precision=prec)
+            got.append((dm.values.numpy(), dm.prec.numpy()))
+        assert np.array_equal(got[0][0], got[1][0]) and np.array_equal(got[0][1], got[1][1])
+        want[prec] = got[0]
+    # both plans in one mixed multi-plan launch, each into outputs of its own
+    dm2 = gpu.DeviceMesh(mesh)
+    for d in (dm, dm2):
+        d.values.fill(np.nan)
+        d.prec.fill(np.nan)
+    gpu.plans_assemble([(p64, dm.coord, dm.coef, dm.values, dm.prec),
+                        (p32, dm2.coord, dm2.coef, dm2.values, dm2.prec)], mf.PREC_MIXED)
+    gpu.synchronize()
+    for d in (dm, dm2):
+        assert np.array_equal(d.values.numpy(), want[mf.PREC_MIXED][0])
+        assert np.array_equal(d.prec.numpy(), want[mf.PREC_MIXED][1])
+    dm.values.fill(7.0)
+    with pytest.raises(MiniFemError) as exc:
+        p32.assemble(dm.coord, dm.coef, dm.values, dm.prec, precision=mf.PREC_FP64)
+    assert exc.value.status == -1
+    jobs = [(p64, dm.coord, dm.coef, dm.values, dm.prec), (p32, dm.coord, dm.coef, dm.values, dm.prec)]
+    with pytest.raises(MiniFemError):
+        gpu.plans_assemble(jobs, mf.PREC_FP64)
+    assert gpu.plans_lds_bytes(jobs, mf.PREC_FP64) is None
+    gpu.synchronize()
+    assert np.all(dm.values.numpy() == 7.0)
 
 
 def test_plan_launched_from_two_host_threads(mf, orc, gpu):

@@ -13,6 +13,7 @@ from minifem.setup import plan_host_stats
     dict(), dict(tile_lds_bytes=8192), dict(tile_lds_bytes=150 * 1024, max_tile_rows=254),
     dict(max_tile_rows=1), dict(use_coord=False), dict(block_threads=512),
     dict(block_threads=1024), dict(id_chunks=3), dict(id_chunks=8, tile_lds_bytes=40000),
+    dict(stage_precision=1), dict(stage_precision=2, tile_lds_bytes=8192),
 ])
 def test_plan_builds_and_validates(mf, n, shuffle, op, params):
     coord, e2n, code, nnz = mf.generate_cube(n, 0.2, 3, shuffle)
@@ -39,9 +40,25 @@ def test_plan_rejects_bad_parameters(mf):
     mesh = mf.Mesh(coord, e2n, code, nnz, 1)
     for bad in (dict(tile_lds_bytes=100), dict(tile_lds_bytes=10 ** 6),
                 dict(max_tile_rows=300), dict(block_threads=128), dict(block_threads=256),
-                dict(block_threads=384)):
+                dict(block_threads=384), dict(stage_precision=3), dict(stage_precision=-1)):
         with pytest.raises(mf.MiniFemError):
             plan_host_stats(mesh, **bad)
+
+
+@pytest.mark.parametrize("op", [0, 1])
+def test_float_stage_sizes_larger_tiles(mf, op):
+    """Tiles sized by the float stage of the mixed modes hold more elements than those
+    sized by the FP64 stage under the same LDS budget: fewer tiles, fewer staged elements
+    (fewer halo copies), and the LDS a workgroup takes is the float layout's, within one
+    CU's. Both mixed modes size alike."""
+    coord, e2n, code, nnz = mf.generate_cube(24, 0.25, 3, 17)
+    mesh = mf.Mesh(coord, e2n, code, nnz, op)
+    s64 = plan_host_stats(mesh, tile_lds_bytes=40000)
+    s32 = plan_host_stats(mesh, tile_lds_bytes=40000, stage_precision=1)
+    assert plan_host_stats(mesh, tile_lds_bytes=40000, stage_precision=2) == s32
+    assert s32["tiles"] < s64["tiles"]
+    assert s32["tile_elems"] < s64["tile_elems"]
+    assert 0 < s32["lds_bytes"] <= 163840 and 0 < s64["lds_bytes"] <= 163840
 
 
 def test_plan_rejects_inconsistent_pattern(mf):

@@ -25,13 +25,14 @@ def test_retime_keeps_payloads():
 
 
 def test_request_descriptor_layout(tmp_path):
-    """mf_request, mf_server_mesh and mf_server_stats in ctypes match include/minifem.h:
-    sizes and field offsets printed by a C program built against the header."""
+    """mf_request, mf_server_mesh, mf_server_stats and mf_plan_params in ctypes match
+    include/minifem.h: sizes and field offsets printed by a C program built against the
+    header."""
     import ctypes
     import os
     import shutil
     import subprocess
-    from minifem._lib import Request as RequestDesc, ServerMesh, ServerStats
+    from minifem._lib import PlanParams, Request as RequestDesc, ServerMesh, ServerStats
     cc = shutil.which("gcc") or shutil.which("cc")
     if cc is None:
         pytest.skip("no C compiler")
@@ -48,6 +49,8 @@ int main(void) {
          offsetof(mf_server_mesh, nprec), offsetof(mf_server_mesh, ncoord),
          offsetof(mf_server_mesh, ncoef));
   printf("%zu %zu\\n", sizeof(mf_server_stats), offsetof(mf_server_stats, makespan_s));
+  printf("%zu %zu %zu\\n", sizeof(mf_plan_params), offsetof(mf_plan_params, id_chunks),
+         offsetof(mf_plan_params, stage_precision));
   return 0;
 }
 """)
@@ -60,3 +63,5 @@ int main(void) {
     assert lines[1] == [ctypes.sizeof(ServerMesh), ServerMesh.nvalues.offset,
                         ServerMesh.nprec.offset, ServerMesh.ncoord.offset, ServerMesh.ncoef.offset]
     assert lines[2] == [ctypes.sizeof(ServerStats), ServerStats.makespan_s.offset]
+    assert lines[3] == [ctypes.sizeof(PlanParams), PlanParams.id_chunks.offset,
+                        PlanParams.stage_precision.offset]
