@@ -90,6 +90,10 @@ def main():
     ap.add_argument("--copy", type=int, default=0,
                     help="instead of the FEM step, time the plain HBM copy kernel on this many "
                          "bytes (bandwidth reference and byte-counter calibration)")
+    ap.add_argument("--pcg", type=int, default=0,
+                    help="instead of the FEM step, time this many iterations of the block-Jacobi "
+                         "PCG (mf_pcg) on the operator and preconditioner the step assembles "
+                         "for the --op/--n cube")
     ap.add_argument("--coloring", action="store_true",
                     help="instead of the FEM step, time the greedy coloring of the --n cube "
                          "(node ids shuffled) on the GPU against the host scan, same result")
@@ -172,6 +176,8 @@ def main():
                           "host_ms": round(t_host[0] * 1e3, 2), "equal": same}), flush=True)
         return
 
+    if args.pcg:
+        return bench_pcg(args)
     if args.batched:
         return bench_batched(args)
     if args.serve:
@@ -539,6 +545,72 @@ def rel_fro(a, b):
     n = float(np.linalg.norm(b[m]))
     d = float(np.linalg.norm(a[m] - b[m]))
     return d / n if n > 0 else d
+
+
+def pcg_iteration_bytes(N, nnz, op):
+    """HBM bytes one PCG iteration of mf_pcg moves on n = N d dofs, each array once: the
+    operator product (values, col_idx, row_ptr, the per-node mask byte, the masked
+    direction it gathers, the direction and the product), the fused update and
+    block-Jacobi step (x, p, q, r and prec in; x, r and z out) and the direction update
+    (z and p in, p and its masked copy out, the mask)."""
+    d = 3 if op == 1 else 1
+    n = N * d
+    spmv = 8 * d * d * nnz + 4 * nnz + 4 * (N + 1) + N + 24 * n
+    return {"spmv": spmv, "total": spmv + (56 * n + 8 * d * d * N) + 32 * n + N}
+
+
+def bench_pcg(args):
+    """K iterations of the block-Jacobi PCG on the bench cube's assembled operator (rtol
+    0: every iteration runs), right-hand side P 1, x0 = 0, in the mesh's own node
+    numbering (mf_pcg) and in the plan's tile order (mf_pcg_ordered). Per iteration:
+    (time of K iterations - time of 0) / K; the call's fixed part (mask, first residual,
+    and for the ordered form the renumbered copy of the system) is the time of 0."""
+    import numpy as np
+
+    import minifem
+    import minifem.gpu as gpu
+    gpu.init(0)
+    op = minifem.OP_ELA if args.op == "ela" else minifem.OP_LAP
+    t0 = time.time()
+    coord, e2n, code, nnz = minifem.generate_cube(args.n, 0.25, 3, 203)
+    mesh = minifem.Mesh(coord, e2n, code, nnz, op, coef=minifem.generate_coef(op, e2n.size // 4, 103))
+    dm = gpu.DeviceMesh(mesh)
+    plan = gpu.Plan(mesh)
+    plan.assemble(dm.coord, dm.coef, dm.values, dm.prec)
+    order = plan.row_order()
+    gpu.synchronize()
+    log(f"pcg: {args.op} {args.n}^3 assembled ({time.time() - t0:.1f}s)")
+    d = 3 if op == minifem.OP_ELA else 1
+    b = gpu.DeviceArray.from_numpy(np.ones(mesh.nbNodes * d))
+    x = gpu.DeviceArray.zeros(mesh.nbNodes * d, np.float64)
+    B = pcg_iteration_bytes(mesh.nbNodes, mesh.nnz, op)
+
+    def run(k, o):
+        x.fill(0.0)
+        gpu.synchronize()
+        t = time.perf_counter()
+        it, rel = gpu.pcg(op, mesh.nbNodes, dm.row, dm.col, dm.values, dm.prec, dm.checkBounds,
+                          b, x, rtol=1e-300, max_iter=k, order=o)
+        return time.perf_counter() - t, it, rel
+
+    forms = {}
+    for name, o in (("mesh_order", None), ("plan_order", order)):
+        run(max(1, args.warmup), o)
+        t_fixed = min(run(0, o)[0] for _ in range(3))
+        t_k, it, rel = min(run(args.pcg, o) for _ in range(2))
+        us = (t_k - t_fixed) / max(it, 1) * 1e6
+        forms[name] = {"us_per_iteration": round(us, 2), "fixed_ms": round(t_fixed * 1e3, 3),
+                       "iterations": it, "rel_residual": rel,
+                       "iteration_GBs": round(B["total"] / us / 1e3, 1),
+                       "frac_of_hbm_peak": round(B["total"] / us / 1e3 / HBM_PEAK_GBS, 4)}
+        log(f"pcg {name}: {forms[name]}")
+    best = forms["plan_order"]
+    out = {"metric": "PCG iterations/s on the assembled operator", "unit": "iterations/s",
+           "value": round(1e6 / best["us_per_iteration"], 1), "form": "plan_order",
+           "workload": f"{args.op} {args.n}^3 node-shuffled: {mesh.nbNodes * d} dofs, "
+                       f"{mesh.nnz} CSR blocks, {args.pcg} iterations",
+           "iteration_bytes": B, "forms": forms}
+    print(json.dumps(out), flush=True)
 
 
 def bench_batched(args):

@@ -253,6 +253,19 @@ int mf_pcg(int operatorID, int nbNodes, const int *row, const int *col, const do
            const double *prec, const int *checkBounds, const double *b, double *x, double rtol,
            int maxIter, int *iterations, double *relResidual, void *stream);
 
+/* mf_pcg on the system renumbered by order (device, nbNodes ints: new node t is node
+ * order[t]; MF_ERR_ARG unless a permutation of 0..nbNodes-1, and for a row_ptr that does
+ * not start at 0 and grow, or a column outside 1..nbNodes). values, col, prec, b and x are
+ * copied into that numbering (temporary device memory of about the size of values), the
+ * solve runs there, and x is copied back. An order that keeps neighbouring nodes close
+ * (mf_plan_row_order) turns the operator product's scattered reads into cache hits on a
+ * node-shuffled mesh. Same iterates as mf_pcg up to the order of the dot products' terms.
+ * No counterpart in the reference, which has no solver. */
+int mf_pcg_ordered(int operatorID, int nbNodes, const int *row, const int *col,
+                   const double *values, const double *prec, const int *checkBounds,
+                   const int *order, const double *b, double *x, double rtol, int maxIter,
+                   int *iterations, double *relResidual, void *stream);
+
 /* compute_double_norm (src/FEM.cc:48-56) on the device: deterministic tree sum of
  * squares, sqrt on the host. x is device memory; blocks until the result is ready. */
 int mf_norm(const double *x, int64_t n, double *result, void *stream);
@@ -322,6 +335,11 @@ int mf_plan_host_stats(int operatorID, int nbElem, int nbNodes, int nnz,
  * device, [5] LDS bytes per workgroup (in the stage precision), [6] threads per
  * workgroup, [7] rows per tile (max). */
 int mf_plan_stats(const mf_plan *plan, int64_t out[8]);
+
+/* The plan's node order (device, N ints, 0-based): the rows of its tiles in tile order
+ * (the rows of a tile are spatially close), then the nodes with an empty row; a
+ * permutation of 0..N-1, for mf_pcg_ordered. No counterpart in the reference. */
+int mf_plan_row_order(const mf_plan *plan, int *order, void *stream);
 
 /* One iteration: values (assembly, src/FEM.cc:183) and the inverted block-Jacobi prec
  * (prec_init + prec_inversion, src/FEM.cc:199-232) in one launch. Device: coord[3N],

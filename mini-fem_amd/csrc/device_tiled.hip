@@ -916,6 +916,23 @@ __global__ void k_prec_isolated(const int32_t *__restrict__ nodes, int n,
     for (int q = 0; q < 9; q++) prec[(int64_t)node * 9 + q] = blk[q];
 }
 
+// The plan's node order: the rows of tile 0, tile 1, ... (each tile's rows are spatially
+// close), then the nodes with an empty row. Tile t's rows sit at order[rinfo_start ..]:
+// the row-info array holds one entry per tile row, in tile order.
+__global__ void k_row_order(const mf::TileHdr *__restrict__ tiles, int ntiles,
+                            const int32_t *__restrict__ tile_nodes,
+                            const int32_t *__restrict__ isolated, int n_isolated, int n_rows,
+                            int *__restrict__ order)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const mf::TileHdr h = tiles[t];
+        for (int r = threadIdx.x; r < h.n_rows; r += blockDim.x)
+            order[h.rinfo_start + r] = tile_nodes[h.node_start + r];
+    }
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n_isolated; k += gridDim.x * blockDim.x)
+        order[n_rows + k] = isolated[k];
+}
+
 // Write-only probe of the assembly's output stream. Mode 0: every tile row's CSR segment,
 // tiles dealt to XCDs and walked as k_tiled walks them, each segment written front to
 // back in 16-byte pieces (the copy-out of phase W); mode 1: the whole value array front to
@@ -1276,6 +1293,19 @@ extern "C" int mf_plan_stats(const mf_plan *plan, int64_t out[8])
 {
     if (!plan || !out) return MF_ERR_ARG;
     std::memcpy(out, plan->stats, sizeof(plan->stats));
+    return MF_OK;
+}
+
+extern "C" int mf_plan_row_order(const mf_plan *plan, int *order, void *stream)
+{
+    if (!plan || (plan->N > 0 && !order)) return MF_ERR_ARG;
+    if (plan->N == 0) return MF_OK;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int grid = std::max(1, std::min(plan->ntiles, 4096));
+    hipLaunchKernelGGL(k_row_order, dim3(grid), dim3(256), 0, s, plan->tiles, plan->ntiles,
+                       plan->tile_nodes, plan->isolated, plan->n_isolated,
+                       plan->N - plan->n_isolated, order);
+    MF_CHECK_LAUNCH();
     return MF_OK;
 }
 

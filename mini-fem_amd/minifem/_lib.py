@@ -82,6 +82,8 @@ SIGNATURES = {
     "mf_dev_permute_int_2d": (_i, [_p, _p, _i, _i, _p]),
     "mf_dev_permute_double_2d": (_i, [_p, _p, _i, _i, _p]),
     "mf_pcg": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p, ctypes.c_double, _i, _p, _p, _p]),
+    "mf_pcg_ordered": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, ctypes.c_double, _i, _p, _p,
+                            _p]),
     "mf_host_alloc": (_i, [_p, ctypes.c_size_t]),
     "mf_host_free": (_i, [_p]),
     "mf_memcpy_h2d_async": (_i, [_p, _p, ctypes.c_size_t, _p]),
@@ -95,6 +97,7 @@ SIGNATURES = {
     "mf_plan_create": (_i, [_p, _i, _i, _i, _i, _p, _p, _p, _p, _p, _p]),
     "mf_plan_destroy": (_i, [_p]),
     "mf_plan_stats": (_i, [_p, _p]),
+    "mf_plan_row_order": (_i, [_p, _p, _p]),
     "mf_plan_assemble": (_i, [_p, _p, _p, _p, _p, _i, _p, _p]),
     "mf_plan_write_probe": (_i, [_p, _p, _i, _p]),
     "mf_plan_host_stats": (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p]),

@@ -306,15 +306,22 @@ def permute_2d(tab, perm, dim, stream=None):
 
 
 def pcg(operator_id, nbNodes, row, col, values, prec, checkBounds, b, x, rtol=1e-10,
-        max_iter=1000, stream=None):
+        max_iter=1000, stream=None, order=None):
     """Block-Jacobi preconditioned CG on the assembled operator (mf_pcg); x is updated in
-    place. Returns (iterations, relative residual)."""
+    place. order (device int32, e.g. Plan.row_order()): solve in that node numbering
+    (mf_pcg_ordered). Returns (iterations, relative residual)."""
     _ensure()
     it = ctypes.c_int()
     rr = ctypes.c_double()
-    check(lib().mf_pcg(int(operator_id), int(nbNodes), _p(row), _p(col), _p(values), _p(prec),
-                       _p(checkBounds), _p(b), _p(x), float(rtol), int(max_iter),
-                       ctypes.byref(it), ctypes.byref(rr), _sp(stream)), "mf_pcg")
+    if order is None:
+        check(lib().mf_pcg(int(operator_id), int(nbNodes), _p(row), _p(col), _p(values), _p(prec),
+                           _p(checkBounds), _p(b), _p(x), float(rtol), int(max_iter),
+                           ctypes.byref(it), ctypes.byref(rr), _sp(stream)), "mf_pcg")
+    else:
+        check(lib().mf_pcg_ordered(int(operator_id), int(nbNodes), _p(row), _p(col), _p(values),
+                                   _p(prec), _p(checkBounds), _p(order), _p(b), _p(x), float(rtol),
+                                   int(max_iter), ctypes.byref(it), ctypes.byref(rr), _sp(stream)),
+              "mf_pcg_ordered")
     return it.value, rr.value
 
 
@@ -417,6 +424,12 @@ class Plan:
         keys = ["tiles", "tile_elems", "tasks", "incidences", "plan_bytes", "lds_bytes",
                 "block_threads", "max_rows"]
         return dict(zip(keys, (int(x) for x in out)))
+
+    def row_order(self, stream=None):
+        """The plan's node order as a device int32 array (mf_plan_row_order)."""
+        out = DeviceArray(max(self.mesh.nbNodes, 1), np.int32)
+        check(self._lib.mf_plan_row_order(self.ptr, out.ptr, _sp(stream)), "mf_plan_row_order")
+        return out
 
     def assemble(self, coord, coef, values, prec, precision=PREC_FP64, singular=None,
                  stream=None):
