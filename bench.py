@@ -345,6 +345,11 @@ def main():
         "tuning": tuning,
     }
     if not args.no_layers and args.n == 100 and args.op == "ela" and plan is not None:
+        # the consumer of the step's outputs, on the operator and prec just assembled
+        if prec_mode == minifem.PREC_FP64:
+            step()
+            stream.sync()
+            out["pcg"] = pcg_forms(mesh, dm, plan, op, 100, 5)
         # the other two workloads of the headline metric, on the same GPU in the same run
         del plan, dm
         out["batched"] = batched_leg(args)
@@ -559,28 +564,18 @@ def pcg_iteration_bytes(N, nnz, op):
     return {"spmv": spmv, "total": spmv + (56 * n + 8 * d * d * N) + 32 * n + N}
 
 
-def bench_pcg(args):
-    """K iterations of the block-Jacobi PCG on the bench cube's assembled operator (rtol
-    0: every iteration runs), right-hand side P 1, x0 = 0, in the mesh's own node
-    numbering (mf_pcg) and in the plan's tile order (mf_pcg_ordered). Per iteration:
-    (time of K iterations - time of 0) / K; the call's fixed part (mask, first residual,
-    and for the ordered form the renumbered copy of the system) is the time of 0."""
+def pcg_forms(mesh, dm, plan, op, iters, warmup):
+    """Block-Jacobi PCG on the operator and prec in dm (rtol 0: every iteration runs),
+    right-hand side P 1, x0 = 0, in the mesh's own node numbering (mf_pcg) and in the
+    plan's tile order (mf_pcg_ordered). Per iteration: (time of `iters` iterations - time of
+    0) / iters; the call's fixed part (mask, first residual, and for the ordered form the
+    renumbered copy of the system) is the time of 0."""
     import numpy as np
 
     import minifem
     import minifem.gpu as gpu
-    gpu.init(0)
-    op = minifem.OP_ELA if args.op == "ela" else minifem.OP_LAP
-    t0 = time.time()
-    coord, e2n, code, nnz = minifem.generate_cube(args.n, 0.25, 3, 203)
-    mesh = minifem.Mesh(coord, e2n, code, nnz, op, coef=minifem.generate_coef(op, e2n.size // 4, 103))
-    dm = gpu.DeviceMesh(mesh)
-    plan = gpu.Plan(mesh)
-    plan.assemble(dm.coord, dm.coef, dm.values, dm.prec)
-    order = plan.row_order()
-    gpu.synchronize()
-    log(f"pcg: {args.op} {args.n}^3 assembled ({time.time() - t0:.1f}s)")
     d = 3 if op == minifem.OP_ELA else 1
+    order = plan.row_order()
     b = gpu.DeviceArray.from_numpy(np.ones(mesh.nbNodes * d))
     x = gpu.DeviceArray.zeros(mesh.nbNodes * d, np.float64)
     B = pcg_iteration_bytes(mesh.nbNodes, mesh.nnz, op)
@@ -595,21 +590,41 @@ def bench_pcg(args):
 
     forms = {}
     for name, o in (("mesh_order", None), ("plan_order", order)):
-        run(max(1, args.warmup), o)
+        run(max(1, warmup), o)
         t_fixed = min(run(0, o)[0] for _ in range(3))
-        t_k, it, rel = min(run(args.pcg, o) for _ in range(2))
+        t_k, it, rel = min(run(iters, o) for _ in range(2))
         us = (t_k - t_fixed) / max(it, 1) * 1e6
         forms[name] = {"us_per_iteration": round(us, 2), "fixed_ms": round(t_fixed * 1e3, 3),
                        "iterations": it, "rel_residual": rel,
                        "iteration_GBs": round(B["total"] / us / 1e3, 1),
                        "frac_of_hbm_peak": round(B["total"] / us / 1e3 / HBM_PEAK_GBS, 4)}
         log(f"pcg {name}: {forms[name]}")
-    best = forms["plan_order"]
+    return {"consumer": "block-Jacobi PCG on the step's operator and prec (mf_pcg, "
+                        "mf_pcg_ordered with the plan's row order)",
+            "iteration_bytes": B, "forms": forms}
+
+
+def bench_pcg(args):
+    """--pcg K: the PCG consumer alone on the --op/--n bench cube (pcg_forms)."""
+    import minifem
+    import minifem.gpu as gpu
+    gpu.init(0)
+    op = minifem.OP_ELA if args.op == "ela" else minifem.OP_LAP
+    t0 = time.time()
+    coord, e2n, code, nnz = minifem.generate_cube(args.n, 0.25, 3, 203)
+    mesh = minifem.Mesh(coord, e2n, code, nnz, op, coef=minifem.generate_coef(op, e2n.size // 4, 103))
+    dm = gpu.DeviceMesh(mesh)
+    plan = gpu.Plan(mesh)
+    plan.assemble(dm.coord, dm.coef, dm.values, dm.prec)
+    gpu.synchronize()
+    log(f"pcg: {args.op} {args.n}^3 assembled ({time.time() - t0:.1f}s)")
+    res = pcg_forms(mesh, dm, plan, op, args.pcg, args.warmup)
+    best = res["forms"]["plan_order"]
+    d = 3 if op == minifem.OP_ELA else 1
     out = {"metric": "PCG iterations/s on the assembled operator", "unit": "iterations/s",
            "value": round(1e6 / best["us_per_iteration"], 1), "form": "plan_order",
            "workload": f"{args.op} {args.n}^3 node-shuffled: {mesh.nbNodes * d} dofs, "
-                       f"{mesh.nnz} CSR blocks, {args.pcg} iterations",
-           "iteration_bytes": B, "forms": forms}
+                       f"{mesh.nnz} CSR blocks, {args.pcg} iterations", **res}
     print(json.dumps(out), flush=True)
 
 
