@@ -12,7 +12,8 @@
 //
 // One iteration is three streaming launches, each reading what it needs once, and two
 // one-workgroup sums of their partials; the scalars stay on the device:
-//   q = K p with the partial sums of p.q          (k_spmv: 16 lanes per row; k_sum_pq)
+//   q = K p with the partial sums of p.q          (k_spmv, k_spmv_ela_staged: 16 lanes
+//                                                   per row; k_sum_pq)
 //   x += alpha p, r -= alpha q, z = M r with the partial sums of r.z and r.r
 //                                                  (k_update_prec_dot; k_sum_rz)
 //   p = z + beta p and its masked copy P p         (k_xpby)
@@ -134,6 +135,83 @@ __global__ void __launch_bounds__(kBT)
             for (int d = 0; d < D; d++) {
                 const double qi = (mi >> d) & 1u ? pi[d] : acc[d];
                 q[(int64_t)i * D + d] = qi;
+                pq += pi[d] * qi;
+            }
+        }
+    }
+    const double t = block_sum(s, pq);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// The ela product with the value stream staged through LDS: a wave's four row groups own
+// four consecutive rows, whose blocks are one contiguous segment of values; the wave
+// reads it in chunks of kChunk blocks with consecutive lanes on consecutive 16-byte
+// pieces (the per-lane 72-byte blocks of k_spmv touch each line from several lanes and
+// several instructions), then every lane takes its blocks from LDS in k_spmv's order, so
+// the sums, and q, are k_spmv's bit for bit. Only this wave reads its LDS region, and LDS
+// operations of one wave complete in order, so no workgroup barrier is needed.
+constexpr int kChunk = 64;
+__global__ void __launch_bounds__(kBT)
+    k_spmv_ela_staged(const int *__restrict__ row, const int *__restrict__ col,
+                      const double *__restrict__ values, const uint8_t *__restrict__ mask, int N,
+                      const double *__restrict__ pm, const double *__restrict__ p,
+                      double *__restrict__ q, double *__restrict__ part, const CgState *__restrict__ cs)
+{
+    if (cs && cs->done) return;
+    __shared__ double s[kBT];
+    __shared__ __align__(16) double stage[kBT / 64][kChunk * 9];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int l = lane & (kGroup - 1), g = lane / kGroup;
+    const int stride = gridDim.x * (kBT / kGroup);
+    double *st = stage[w];
+    double pq = 0.0;
+    for (int r0 = blockIdx.x * (kBT / kGroup) + w * (64 / kGroup); r0 < N; r0 += stride) {
+        const int i = r0 + g;
+        const bool on = i < N;
+        const int ic = on ? i : r0;
+        const int q0 = row[ic], q1 = on ? row[ic + 1] : q0;
+        const int s0 = row[r0], s1 = row[r0 + (64 / kGroup) < N ? r0 + (64 / kGroup) : N];
+        const unsigned mi = mask[ic];
+        double pi[3], acc[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            pi[d] = p[(int64_t)ic * 3 + d];
+            acc[d] = 0.0;
+        }
+        int k = q0 + l;  // this lane's next block: q0 + l, q0 + l + 16, ...
+        for (int c0 = s0; c0 < s1; c0 += kChunk) {
+            const int nb = s1 - c0 < kChunk ? s1 - c0 : kChunk, nd = nb * 9;
+            const double *src = values + (int64_t)c0 * 9;
+            for (int d = 2 * lane; d < nd; d += 128) {
+                if (d + 1 < nd) {
+                    const double2 v = *reinterpret_cast<const double2 *>(src + d);
+                    st[d] = v.x;
+                    st[d + 1] = v.y;
+                } else {
+                    st[d] = src[d];
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (; k < q1 && k < c0 + nb; k += kGroup) {
+                const int j = col[k] - 1;
+                const double *b = st + (k - c0) * 9;
+                const double x0 = pm[(int64_t)j * 3], x1 = pm[(int64_t)j * 3 + 1],
+                             x2 = pm[(int64_t)j * 3 + 2];
+#pragma unroll
+                for (int d = 0; d < 3; d++)
+                    acc[d] += b[3 * d] * x0 + b[3 * d + 1] * x1 + b[3 * d + 2] * x2;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+#pragma unroll
+        for (int o = kGroup / 2; o > 0; o >>= 1)
+#pragma unroll
+            for (int d = 0; d < 3; d++) acc[d] += __shfl_xor(acc[d], o, kGroup);
+        if (l == 0 && on) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double qi = (mi >> d) & 1u ? pi[d] : acc[d];
+                q[(int64_t)i * 3 + d] = qi;
                 pq += pi[d] * qi;
             }
         }
@@ -433,8 +511,8 @@ static int pcg_core(bool ela, int nbNodes, const int *row, const int *col, const
     // out = K in, from in and its masked copy in_m (state: skipped once the solve is done)
     auto spmv = [&](const double *in_m, const double *in, double *out, const CgState *c) {
         if (ela)
-            hipLaunchKernelGGL(k_spmv<true>, dim3(gSpmv), dim3(kBT), 0, s, row, col, values, mask,
-                               nbNodes, in_m, in, out, partA, c);
+            hipLaunchKernelGGL(k_spmv_ela_staged, dim3(gSpmv), dim3(kBT), 0, s, row, col, values,
+                               mask, nbNodes, in_m, in, out, partA, c);
         else
             hipLaunchKernelGGL(k_spmv<false>, dim3(gSpmv), dim3(kBT), 0, s, row, col, values, mask,
                                nbNodes, in_m, in, out, partA, c);
