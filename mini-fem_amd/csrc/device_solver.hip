@@ -36,7 +36,10 @@
 namespace {
 
 constexpr int kBT = 256;          // threads per workgroup
-constexpr int kGroup = 16;        // lanes per row in the operator product
+constexpr int kGroup = 16;        // lanes per row in the operator product (ela)
+// lanes per row of the lap product: 8 took 82 against 87 us per lap 100^3 iteration in the
+// plan's order but 217 against 204 us in the shuffled one, 4 was slower in both
+constexpr int kLapGroup = 16;
 constexpr int kMaxGrid = 2048;    // workgroups of the reducing kernels (partials per sum)
 constexpr int kBatch = 8;         // iterations enqueued between two looks at the state
 
@@ -84,7 +87,7 @@ __device__ __forceinline__ double block_sum(double *s, double v)
 // share of p.q. The columns are gathered from pm = P p (fixed dofs already zero), so a
 // block costs its own loads and one gather of pm; a fixed row takes p's own value. Every
 // lane of a row group runs the same trip count, so the butterfly always has its 16 lanes.
-template <bool ELA>
+template <bool ELA, int G>
 __global__ void __launch_bounds__(kBT)
     k_spmv(const int *__restrict__ row, const int *__restrict__ col,
            const double *__restrict__ values, const uint8_t *__restrict__ mask, int N,
@@ -94,10 +97,10 @@ __global__ void __launch_bounds__(kBT)
     if (cs && cs->done) return;  // the whole grid returns: nothing waits on the butterfly
     __shared__ double s[kBT];
     constexpr int D = ELA ? 3 : 1;
-    const int l = threadIdx.x % kGroup;
-    const int groups = gridDim.x * (kBT / kGroup);
+    const int l = threadIdx.x % G;
+    const int groups = gridDim.x * (kBT / G);
     double pq = 0.0;
-    for (int i = blockIdx.x * (kBT / kGroup) + threadIdx.x / kGroup; i < N; i += groups) {
+    for (int i = blockIdx.x * (kBT / G) + threadIdx.x / G; i < N; i += groups) {
         const int q0 = row[i], q1 = row[i + 1];
         const unsigned mi = mask[i];
         double pi[D], acc[D];
@@ -106,7 +109,7 @@ __global__ void __launch_bounds__(kBT)
             pi[d] = p[(int64_t)i * D + d];
             acc[d] = 0.0;
         }
-        for (int k = q0 + l; k < q1; k += kGroup) {
+        for (int k = q0 + l; k < q1; k += G) {
             const int j = col[k] - 1;
             const double *b = values + (int64_t)k * D * D;
             if constexpr (ELA) {
@@ -127,9 +130,9 @@ __global__ void __launch_bounds__(kBT)
             }
         }
 #pragma unroll
-        for (int o = kGroup / 2; o > 0; o >>= 1)
+        for (int o = G / 2; o > 0; o >>= 1)
 #pragma unroll
-            for (int d = 0; d < D; d++) acc[d] += __shfl_xor(acc[d], o, kGroup);
+            for (int d = 0; d < D; d++) acc[d] += __shfl_xor(acc[d], o, G);
         if (l == 0) {
 #pragma unroll
             for (int d = 0; d < D; d++) {
@@ -502,6 +505,7 @@ static int pcg_core(bool ela, int nbNodes, const int *row, const int *col, const
     CgState *cs = reinterpret_cast<CgState *>(sums + 2);
     uint8_t *mask = reinterpret_cast<uint8_t *>(cs + 1);
     const int gSpmv = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxGrid, ((int64_t)nbNodes * kGroup + kBT - 1) / kBT));
+    const int gSpmvLap = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxGrid, ((int64_t)nbNodes * kLapGroup + kBT - 1) / kBT));
     const int gNode = grid_for(nbNodes), gDof = grid_for(n);
     int st = MF_OK;
     auto launched = [&]() {
@@ -514,8 +518,8 @@ static int pcg_core(bool ela, int nbNodes, const int *row, const int *col, const
             hipLaunchKernelGGL(k_spmv_ela_staged, dim3(gSpmv), dim3(kBT), 0, s, row, col, values,
                                mask, nbNodes, in_m, in, out, partA, c);
         else
-            hipLaunchKernelGGL(k_spmv<false>, dim3(gSpmv), dim3(kBT), 0, s, row, col, values, mask,
-                               nbNodes, in_m, in, out, partA, c);
+            hipLaunchKernelGGL((k_spmv<false, kLapGroup>), dim3(gSpmvLap), dim3(kBT), 0, s, row, col,
+                               values, mask, nbNodes, in_m, in, out, partA, c);
     };
     // p = z (c NULL) or p = z + beta p, and pm = P p
     auto xpby = [&](const CgState *c) {
@@ -582,7 +586,7 @@ static int pcg_core(bool ela, int nbNodes, const int *row, const int *col, const
     while (st == MF_OK && !h.done) {
         for (int k = 0; k < kBatch; k++) {
             spmv(pm, p, q, cs);
-            hipLaunchKernelGGL(k_sum_pq, dim3(1), dim3(kBT), 0, s, partA, gSpmv, cs);
+            hipLaunchKernelGGL(k_sum_pq, dim3(1), dim3(kBT), 0, s, partA, ela ? gSpmv : gSpmvLap, cs);
             if (ela)
                 hipLaunchKernelGGL(k_update_prec_dot<true>, dim3(gNode), dim3(kBT), 0, s, cs, p, q,
                                    x, r, prec, mask, nbNodes, z, partA, partB);
