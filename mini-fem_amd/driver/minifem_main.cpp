@@ -12,6 +12,11 @@
 //     --precision P       fp64 | mixed; default fp64
 //     --coef FILE         per-element coefficients (raw float64, file element order)
 //     --store-check       write checkings/<op>_1_0 from this run instead of reading it
+//     --solve RTOL        after the loop, the "solver" of the reference's main loop
+//                         (src/main.cc:353, not implemented there): block-Jacobi PCG on
+//                         the last iteration's operator and prec, K x = P 1 from x = 0 to
+//                         RTOL (mf_pcg_ordered in the plan's node order when tiled, mf_pcg
+//                         otherwise)
 //   minifem gen <mesh> <lap|ela> <n> <jitter> <geomSeed> <shuffleSeed> [--data DIR]
 //     writes DIR/<mesh>/inputs/<op>_1_0 for an n^3 jittered Kuhn cube
 //
@@ -121,12 +126,14 @@ int main(int argc, char **argv)
     const int nbIter = (int)std::strtol(argv[3], nullptr, 0);
     if (nbIter < 1) die("Number of iterations must be at least 1.");
     std::string schedule = "tiled", precName = "fp64", coefPath;
+    double solveTol = 0.0;  // > 0: solve after the loop
     bool storeCheck = false;
     for (int i = 4; i < argc; i++) {
         if (!std::strcmp(argv[i], "--schedule") && i + 1 < argc) schedule = argv[++i];
         else if (!std::strcmp(argv[i], "--precision") && i + 1 < argc) precName = argv[++i];
         else if (!std::strcmp(argv[i], "--coef") && i + 1 < argc) coefPath = argv[++i];
         else if (!std::strcmp(argv[i], "--store-check")) storeCheck = true;
+        else if (!std::strcmp(argv[i], "--solve") && i + 1 < argc) solveTol = std::atof(argv[++i]);
         else if (!std::strcmp(argv[i], "--data")) i++;
         else die(std::string("unknown option ") + argv[i]);
     }
@@ -260,6 +267,34 @@ int main(int argc, char **argv)
                 1e3 * phase_ms[0] / timed, 1e3 * phase_ms[1] / timed, 1e3 * phase_ms[2] / timed,
                 1e3 * phase_ms[3] / timed,
                 1e3 * (phase_ms[0] + phase_ms[1] + phase_ms[2] + phase_ms[3]) / timed);
+
+    if (solveTol > 0.0) {
+        std::printf("Solving K x = P 1 (block-Jacobi PCG)...");
+        std::fflush(stdout);
+        const size_t n = (size_t)nN * (dim == 9 ? 3 : 1);
+        double *dB = dev_upload(std::vector<double>(n, 1.0));
+        double *dX = dev_upload(std::vector<double>(n, 0.0));
+        int *dOrder = nullptr;
+        int iters = 0;
+        double rel = 0.0;
+        t = clk::now();
+        if (tiled) {
+            hipck(hipMalloc(reinterpret_cast<void **>(&dOrder), std::max<size_t>(1, nN) * sizeof(int)),
+                  "hipMalloc");
+            check(mf_plan_row_order(plan, dOrder, stream), "mf_plan_row_order");
+            check(mf_pcg_ordered(opID, nN, dRow, dCol, dValues, dPrec, dCb, dOrder, dB, dX, solveTol,
+                                 100000, &iters, &rel, stream),
+                  "mf_pcg_ordered");
+        } else {
+            check(mf_pcg(opID, nN, dRow, dCol, dValues, dPrec, dCb, dB, dX, solveTol, 100000, &iters,
+                         &rel, stream),
+                  "mf_pcg");
+        }
+        std::printf(" done  (%.1e seconds, %d iterations, relative residual %.3e)\n\n", secs(t),
+                    iters, rel);
+        for (void *p : {(void *)dB, (void *)dX, (void *)dOrder})
+            if (p) hipck(hipFree(p), "hipFree");
+    }
 
     // Numerical check (src/FEM.cc:59-98): the norms in the reference's serial order
     std::vector<double> hValues((size_t)nnz * dim), hPrec((size_t)nN * dim);

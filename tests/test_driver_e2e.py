@@ -69,12 +69,16 @@ def test_driver_end_to_end_matches_oracle(tmp_path, mf, op):
     for schedule in ("tiled", "colored"):
         work = tmp_path / f"run_{schedule}"
         work.mkdir()
-        out = run([MINIFEM, "CUBE12", op, "3", "--schedule", schedule, "--data", data],
-                  cwd=work).stdout
+        # --solve: the solver the reference's main loop names (src/main.cc:353), on the
+        # last iteration's operator and prec; the numerical check is unchanged by it
+        out = run([MINIFEM, "CUBE12", op, "3", "--schedule", schedule, "--solve", "1e-9",
+                   "--data", data], cwd=work).stdout
         assert f"Schedule               :  {schedule}, fp64" in out
         if schedule == "tiled":
             assert "Building the tiled plan" in out
         assert differences(work / "numerical_results_0") == [0.0, 0.0], (schedule, out)
+        it, rel = re.search(r"done .*?, (\d+) iterations, relative residual (\S+)\)", out).groups()
+        assert 0 < int(it) < 100000 and float(rel) <= 1e-9, out
 
     # the driver's own checkings (--store-check) hold the oracle's norms exactly
     data2 = tmp_path / "data2"
