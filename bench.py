@@ -601,6 +601,10 @@ def pcg_forms(mesh, dm, plan, op, iters, warmup):
         log(f"pcg {name}: {forms[name]}")
     return {"consumer": "block-Jacobi PCG on the step's operator and prec (mf_pcg, "
                         "mf_pcg_ordered with the plan's row order)",
+            "rel_residual": "||P 1 - K x|| / ||P 1|| after the timed iterations; CG minimizes "
+                            "the error's energy norm, and the residual's 2-norm first grows (on "
+                            "the 20^3 ela cube, 16x at iteration 5 before it falls to 1e-9 at "
+                            "400)",
             "iteration_bytes": B, "forms": forms}
 
 
