@@ -442,7 +442,8 @@ int mf_server_add_mesh(mf_server *server, const mf_server_mesh *mesh);
 /* Serves n requests (any order of t_arrival) and returns when all are done: t_start[i]
  * (admission) and t_done[i] (the end of its launch and copies, as the loop saw it), in
  * seconds after the start. MF_ERR_BUDGET, before anything runs, when a request needs
- * more than the arena; MF_ERR_ARG for a bad mesh, kind, mode or payload size. */
+ * more than the arena; MF_ERR_ARG for a bad mesh, kind, mode or payload size, or a mode
+ * the mesh's plan does not launch in (an FP64 request on a plan staged in float). */
 int mf_server_run(mf_server *server, const mf_request *reqs, int n, double *t_start,
                   double *t_done, mf_server_stats *stats);
 /* One run at a time per server (the loop is single-threaded; two threads running one

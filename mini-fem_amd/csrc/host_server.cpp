@@ -210,6 +210,9 @@ extern "C" int mf_server_run(mf_server *server, const mf_request *reqs, int n, d
             const int64_t want = r.kind == MF_REQ_COORD ? 8 * m.ncoord
                                  : r.mode == MF_PREC_MIXED_F32COEF ? 4 * m.ncoef : 8 * m.ncoef;
             if (r.payload_bytes != want) return MF_ERR_ARG;  // the kernel reads exactly this
+            // the mesh's plan must launch in this mode (a plan staged in float serves the
+            // mixed modes only), checked before anything runs
+            if (!S.group_fits(r.mode, {r.mesh})) return MF_ERR_ARG;
             if (S.footprint(r) > S.arena.size()) return MF_ERR_BUDGET;
         }
         std::vector<int> pending((size_t)n);

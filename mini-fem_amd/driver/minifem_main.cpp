@@ -133,7 +133,12 @@ int main(int argc, char **argv)
         else if (!std::strcmp(argv[i], "--precision") && i + 1 < argc) precName = argv[++i];
         else if (!std::strcmp(argv[i], "--coef") && i + 1 < argc) coefPath = argv[++i];
         else if (!std::strcmp(argv[i], "--store-check")) storeCheck = true;
-        else if (!std::strcmp(argv[i], "--solve") && i + 1 < argc) solveTol = std::atof(argv[++i]);
+        else if (!std::strcmp(argv[i], "--solve") && i + 1 < argc) {
+            char *end = nullptr;
+            solveTol = std::strtod(argv[++i], &end);
+            if (end == argv[i] || *end || !(solveTol > 0.0))
+                die(std::string("--solve needs a positive tolerance, got \"") + argv[i] + "\"");
+        }
         else if (!std::strcmp(argv[i], "--data")) i++;
         else die(std::string("unknown option ") + argv[i]);
     }

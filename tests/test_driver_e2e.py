@@ -47,7 +47,8 @@ def binaries():
 def test_driver_rejects_unknown_schedule_and_precision(tmp_path):
     """Argument errors end the driver before any GPU call, with a message."""
     binaries()
-    for opt in (["--schedule", "tiledd"], ["--precision", "fp16"], ["--bogus"]):
+    for opt in (["--schedule", "tiledd"], ["--precision", "fp16"], ["--bogus"],
+                ["--solve", "abc"], ["--solve", "0"]):
         p = run([MINIFEM, "CUBE", "ela", "1", *opt, "--data", tmp_path], check=False)
         assert p.returncode != 0
         assert "Error:" in p.stderr, p.stderr

@@ -209,6 +209,31 @@ def test_serving_rejects_bad_requests(mf, gpu):
     assert stats["requests"] == 2 and srv.arena_used() == 0
 
 
+def test_serving_rejects_fp64_on_float_staged_plans(mf, gpu):
+    """A mixed-precision server sizes its plans' tiles by the float stage, and such a plan
+    does not launch in FP64: the native loop rejects an FP64 request before it serves
+    any, the arena stays empty, and the mixed requests are still served."""
+    import ctypes
+    from minifem._lib import MF_ERR_ARG, PREC_FP64, PREC_MIXED, REQ_COEF, Request as RequestDesc
+    from minifem.serve import Request, Server
+    m = mf.Mesh.cube(10, 0.2, 3, mf.OP_ELA, coef_seed=9)
+    srv = Server([m], hbm_budget=1 << 26, slots=2, precision=PREC_MIXED)
+    assert srv.plans[0].stage_precision == PREC_MIXED
+    reqs = [Request(0, 0.0, 0, "coef", m.coef), Request(1, 0.0, 0, "coef", m.coef)]
+    f64 = gpu.HostArray(m.coef.size, np.float64)
+    f64.array[:] = m.coef
+    lib = mf.lib()
+    arr = (RequestDesc * 1)(RequestDesc(0, REQ_COEF, PREC_FP64, 0, f64.ptr, f64.nbytes, 0.0,
+                                        None, None))
+    ts, td = np.zeros(1), np.zeros(1)
+    assert lib.mf_server_run(srv._srv, ctypes.cast(arr, ctypes.c_void_p), 1, ts.ctypes.data,
+                             td.ctypes.data, None) == MF_ERR_ARG
+    assert srv.arena_used() == 0
+    srv.stage(reqs)
+    stats = srv.run(reqs)
+    assert stats["requests"] == 2 and srv.arena_used() == 0
+
+
 @pytest.mark.parametrize("max_bypass", [0, 3])
 def test_serving_bypass_is_bounded(mf, gpu, max_bypass):
     """Admission policy of the native loop. A large request that fits the arena only when
