@@ -358,7 +358,7 @@ def main():
 
 
 def host_cpu():
-    """Model name and the cores this process may run on (the GPU box's CPU share)."""
+    """Model name and the cores this process may run on (this host's CPU share)."""
     model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
