@@ -379,7 +379,9 @@ def plans_assemble(jobs, precision=PREC_FP64, stream=None):
 
 
 def plans_lds_bytes(jobs, precision=PREC_FP64):
-    """LDS bytes per workgroup of that launch, or None when the jobs do not fit one CU."""
+    """LDS bytes per workgroup of that launch, or None when the jobs cannot run as one
+    launch in this mode (their tiles do not fit one CU together, their workgroup sizes
+    differ, or a plan sized for the float stage is asked for FP64)."""
     arr, n = _job_table(jobs)
     r = lib().mf_plans_lds_bytes(ctypes.cast(arr, ctypes.c_void_p), n, int(precision))
     if r == -1:
