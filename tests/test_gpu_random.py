@@ -1,9 +1,10 @@
-"""Seeded randomized parity of the fused plan against the oracle, bit for bit: cubes of
+"""Seeded randomized parity of the fused plan against the oracle, bit for bit. Cubes of
 random size and jitter, node ids shuffled or not, the element order shuffled or not (so
 coloring, permutation and tiles all change), with or without per-element coefficients,
 under random tile schedules, in FP64 and mixed precision (against the oracle's mixed
-restatement). The FP64 plans that fit one launch together then run as one multi-plan
-launch, each with the bits of its own launch."""
+restatement); the FP64 plans that fit one launch together then run as one multi-plan
+launch, each with the bits of its own launch. Unstructured meshes: Delaunay
+tetrahedralizations of random points (node valences up to ~90)."""
 import ctypes
 
 import numpy as np
@@ -88,3 +89,56 @@ def test_random_meshes_bit_exact(mf, orc, gpu, seed):
     gpu.synchronize()
     for pl, d, v, p in together:
         assert np.array_equal(d.values.numpy(), v) and np.array_equal(d.prec.numpy(), p)
+
+
+def delaunay_mesh(mf, orc, npts, seed, op):
+    """An unstructured tet mesh: the Delaunay tetrahedralization of random points in the
+    unit cube (nearly flat tets dropped), hull nodes with random boundary codes, random
+    coefficients. Valences and colors vary from node to node, unlike the Kuhn cubes."""
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(seed)
+    pts = rng.random((npts, 3))
+    tri = Delaunay(pts)
+    tets = tri.simplices.astype(np.int64)
+    a, b, c = (pts[tets[:, k]] - pts[tets[:, 3]] for k in range(3))
+    tets = tets[np.abs(np.einsum('ij,ij->i', a, np.cross(b, c))) > 1e-9]
+    i, j = tets[:, [0, 0, 0, 1, 1, 2]].ravel(), tets[:, [1, 2, 3, 2, 3, 3]].ravel()
+    edges = np.unique(np.minimum(i, j) * npts + np.maximum(i, j))
+    nnz = npts + 2 * edges.size
+    code = np.zeros(npts, np.int32)
+    hull = np.unique(tri.convex_hull)
+    code[hull] = rng.choice([10, 52, 53, 54], hull.size)
+    coord = pts.ravel().copy()
+    e2n = (tets + 1).astype(np.int32).ravel()
+    coef = mf.generate_coef(op, tets.shape[0], seed + 1)
+    valence = int(np.bincount(tets.ravel()).max())
+    return (mf.Mesh(coord, e2n, code, nnz, op, coef=coef),
+            orc.OracleMesh(coord, e2n, code, nnz, op, coef=coef), valence)
+
+
+@pytest.mark.parametrize("op", [0, 1])
+@pytest.mark.parametrize("npts,seed", [(3000, 5), (20000, 6)])
+def test_delaunay_meshes_bit_exact(mf, orc, gpu, op, npts, seed):
+    """Unstructured meshes, bit for bit against the oracle: the fused plan in FP64 and
+    mixed precision, and the drop-in colored path."""
+    mesh, om, valence = delaunay_mesh(mf, orc, npts, seed, op)
+    print(f"[delaunay] op={op} points={npts} seed={seed} tets={mesh.nbElem} nnz={mesh.nnz} "
+          f"max valence {valence}", flush=True)
+    ov = om.assemble()
+    op_ = om.prec(ov)[0]
+    plan = gpu.Plan(mesh)
+    dm = gpu.DeviceMesh(mesh)
+    dm.values.fill(np.nan)
+    dm.prec.fill(np.nan)
+    plan.assemble(dm.coord, dm.coef, dm.values, dm.prec)
+    assert np.array_equal(dm.values.numpy(), ov)
+    assert same_prec(dm.prec.numpy(), op_)
+    dm.values.fill(np.nan)
+    dm.prec.fill(np.nan)
+    plan.assemble(dm.coord, dm.coef, dm.values, dm.prec, precision=mf.PREC_MIXED)
+    ovm = om.assemble(mixed=True)
+    assert np.array_equal(dm.values.numpy(), ovm)
+    assert same_prec(dm.prec.numpy(), om.prec(ovm)[0])
+    dm.iterate_colored()
+    assert np.array_equal(dm.values.numpy(), ov)
+    assert same_prec(dm.prec.numpy(), op_)
