@@ -46,6 +46,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def delaunay_arrays(npts, seed=5):
+    """An unstructured mesh: the Delaunay tetrahedralization of npts seeded random points in
+    the unit cube (nearly flat tets dropped), hull nodes with seeded boundary codes. Returns
+    (coord, elemToNode 1-based, boundNodesCode, nnz) like minifem.generate_cube."""
+    import numpy as np
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(seed)
+    pts = rng.random((npts, 3))
+    tri = Delaunay(pts)
+    tets = tri.simplices.astype(np.int64)
+    a, b, c = (pts[tets[:, k]] - pts[tets[:, 3]] for k in range(3))
+    tets = tets[np.abs(np.einsum("ij,ij->i", a, np.cross(b, c))) > 1e-9 / npts]
+    i, j = tets[:, [0, 0, 0, 1, 1, 2]].ravel(), tets[:, [1, 2, 3, 2, 3, 3]].ravel()
+    nnz = npts + 2 * np.unique(np.minimum(i, j) * npts + np.maximum(i, j)).size
+    code = np.zeros(npts, np.int32)
+    hull = np.unique(tri.convex_hull)
+    code[hull] = rng.choice([10, 52, 53, 54], hull.size)
+    return pts.ravel().copy(), (tets + 1).astype(np.int32).ravel(), code, int(nnz)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,6 +76,9 @@ def main():
     ap.add_argument("--op", default="ela", choices=["ela", "lap"])
     ap.add_argument("--n", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--delaunay", type=int, default=0,
+                    help="instead of the cube: the Delaunay tetrahedralization of this many "
+                         "seeded random points in the unit cube (an unstructured mesh)")
     ap.add_argument("--no-layers", action="store_true",
                     help="leave out the batched and served legs of the default line")
     ap.add_argument("--cpu-iters", type=int, default=10,
@@ -187,7 +210,12 @@ def main():
     prec_mode = minifem.PREC_FP64 if args.precision == "fp64" else minifem.PREC_MIXED
     gpu.init(0)
     t0 = time.time()
-    coord, e2n, code, nnz = minifem.generate_cube(args.n, 0.25, 3, 203)
+    if args.delaunay:
+        coord, e2n, code, nnz = delaunay_arrays(args.delaunay)
+        shape = f"Delaunay mesh of {args.delaunay} random points"
+    else:
+        coord, e2n, code, nnz = minifem.generate_cube(args.n, 0.25, 3, 203)
+        shape = f"{args.n}^3 jittered Kuhn cube, node-shuffled"
     coef = minifem.generate_coef(op, e2n.size // 4, 103)
     mesh = minifem.Mesh(coord, e2n, code, nnz, op, coef=coef)
     log(f"setup: {mesh.nbNodes} nodes, {mesh.nbElem} tets, {mesh.nnz} blocks, "
@@ -231,7 +259,7 @@ def main():
             if mode < 2:  # modes 2 and 3 write a part of the bytes each
                 res[name]["TB_per_s"] = round(out_bytes / us / 1e6, 3)
         print(json.dumps({"write_probe": res, "bytes": out_bytes,
-                          "workload": f"{args.op} {args.n}^3 node-shuffled, tiled plan "
+                          "workload": f"{args.op} {shape}, tiled plan "
                                       f"{plan.stats()['tiles']} tiles"}), flush=True)
         return
 
@@ -276,7 +304,8 @@ def main():
     # --pmc summary of the same kernel on the same workload (FETCH_SIZE + WRITE_SIZE per
     # dispatch, scaled by the copy-kernel calibration the summary states), labelled so
     traffic = traffic_src = None
-    tname = f"traffic_{args.op}_{args.n}_{args.schedule}_{args.precision}.json"
+    size = f"delaunay{args.delaunay}" if args.delaunay else str(args.n)
+    tname = f"traffic_{args.op}_{size}_{args.schedule}_{args.precision}.json"
     tpath = os.path.join(ROOT, "profiles", tname)
     if os.path.exists(tpath):
         with open(tpath) as f:
@@ -332,7 +361,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if prec_mode == minifem.PREC_FP64 else "f32 products, f64 accumulate",
         "data": "synthetic",
-        "config": {"workload": f"{args.op} {args.n}^3 jittered Kuhn cube, node-shuffled: "
+        "config": {"workload": f"{args.op} {shape}: "
                                f"{mesh.nbNodes} nodes, {mesh.nbElem} tets, {mesh.nnz} CSR blocks; "
                                "assembly + block-Jacobi prec per step",
                    "schedule": args.schedule, "precision": args.precision,
@@ -344,7 +373,8 @@ def main():
         "accuracy": accuracy,
         "tuning": tuning,
     }
-    if not args.no_layers and args.n == 100 and args.op == "ela" and plan is not None:
+    if (not args.no_layers and not args.delaunay and args.n == 100 and args.op == "ela" and
+            plan is not None):
         # the consumer of the step's outputs, on the operator and prec just assembled
         if prec_mode == minifem.PREC_FP64:
             step()
@@ -407,7 +437,8 @@ def cpu_baselines(mesh, op, gpu_value, args):
     b = runs[best]
     cpu = {"value": b["value"], "unit": "CSR blocks/s", "cores": b["cores"], "kind": "port",
            "sample": f"the oracle's OpenMP COLORING iteration ({best}, {b['cores']} threads, "
-                     f"-O2), {b['iterations']} iterations of the same {args.n}^3 {args.op} "
+                     f"-O2), {b['iterations']} iterations of the same "
+                     f"{f'{args.delaunay}-point Delaunay' if args.delaunay else f'{args.n}^3'} {args.op} "
                      f"mesh, iteration 0 excluded: {b['s_per_iteration']} s per iteration "
                      f"(assembly + prec_init + prec_inversion)",
            "gpu_speedup": b["gpu_speedup"], "gpu_speedup_vs_ref_1_core": runs["ref"]["gpu_speedup"],
