@@ -34,3 +34,20 @@ def test_rel_fro():
     got = np.array([3.0, 4.0 + 5e-3, np.inf, 0.0])
     assert abs(b.rel_fro(got, ref) - 1e-3) < 1e-15  # inf entries are left out
     assert b.rel_fro(np.array([1e-3]), np.array([0.0])) == 1e-3  # absolute when ref is 0
+
+
+def test_delaunay_workload_mesh(mf):
+    """bench.py --delaunay: the nnz it returns is the exact CSR size the setup builds (the
+    diagonal plus both directions of every edge), every point is used, and only hull
+    nodes carry boundary codes."""
+    from scipy.spatial import Delaunay
+    b = load_bench()
+    coord, e2n, code, nnz = b.delaunay_arrays(2000, seed=3)
+    n = coord.size // 3
+    assert n == 2000 and e2n.min() == 1 and e2n.max() == n
+    assert np.unique(e2n).size == n
+    mesh = mf.Mesh(coord, e2n, code, nnz, mf.OP_ELA)
+    assert mesh.nnz == nnz
+    hull = np.unique(Delaunay(coord.reshape(-1, 3)).convex_hull)
+    assert set(np.flatnonzero(code)) <= set(hull)
+    assert set(np.unique(code[hull])) <= {10, 52, 53, 54}
